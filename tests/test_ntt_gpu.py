@@ -1,0 +1,175 @@
+"""GPU parity of libntt.so against the oracle (bit-exact, canonical Montgomery).
+
+Mirrors core/tests/ntt.rs: test_ssip_ntt (:26-148: random omega that is not a
+root of unity, GPU poly rotated by 3 and sliced out of a 2^(k+1) buffer, k up
+to 20), test_distribute_zeta (:261-388: [1, zeta, zeta^2] distribution on a
+zero-padded 4x extended domain), plus the workload sizes 2^20 / 2^22 / 2^24
+(forward, inverse, coset g = 7) and the 4x extended coset of 2^22.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from helpers import cpu2gpu, device_array, download, gpu2cpu, upload
+from zkpoly_amd import hip
+from zkpoly_amd.field import FR_MODULUS, mont_limbs, omega, random_elements, to_mont
+from zkpoly_amd.ntt import DistributePowers, SsipNtt, SsipPrecompute, gen_powers
+from zkpoly_amd.runtime import ScalarArray
+
+pytestmark = pytest.mark.gpu
+THREADS = os.cpu_count() or 1
+
+
+def twiddle_table(libs, w_mont, n):
+    """Host twiddles through ssip_precompute, then uploaded (as the compiler does)."""
+    host = np.zeros((n // 2, 4), dtype=np.uint64)
+    arr = ScalarArray(n // 2, host.ctypes.data, device="cpu")
+    SsipPrecompute(libs).get_fn()(arr, w_mont)
+    return host
+
+
+def gpu_ntt(libs, stream, data, w_mont, rotate=0, sliced=False):
+    n = data.shape[0]
+    tw_host = twiddle_table(libs, w_mont, n)
+    tw_buf, tw = upload(tw_host)
+    if sliced:
+        buf, whole = device_array(2 * n)
+        whole.rotate(rotate)
+        x = whole.slice(n, 2 * n)
+    else:
+        buf, x = device_array(n)
+        if rotate:
+            x.rotate(rotate)
+    cpu2gpu(data, x, buf)
+    SsipNtt(libs).get_fn().f([x], [tw, stream], lambda i: i)
+    stream.sync()
+    return gpu2cpu(x, buf), tw_host
+
+
+def test_ssip_precompute_matches_oracle(libs):
+    for k in (1, 2, 5, 11, 16):
+        w = orc.omega(k)
+        got = twiddle_table(libs, w, 1 << k)
+        assert np.array_equal(got, orc.twiddles(1 << (k - 1), w))
+
+
+@pytest.mark.parametrize("k", list(range(1, 21)))
+def test_ssip_ntt_random_omega_rotated_slice(libs, stream, k):
+    """core/tests/ntt.rs:26-148 with a seeded (not OS) RNG."""
+    n = 1 << k
+    data = random_elements(n, 1000 + k)
+    w = random_elements(1, 5000 + k)[0]  # almost surely not a root of unity
+    got, _ = gpu_ntt(libs, stream, data, w, rotate=3, sliced=True)
+    want = orc.best_fft(data, w, k, THREADS)
+    assert np.array_equal(got, want), f"k={k}"
+
+
+@pytest.mark.parametrize("k,rot", [(4, 2), (10, 1023), (13, 5)])
+def test_ssip_ntt_rotated_true_root(libs, stream, k, rot):
+    n = 1 << k
+    data = random_elements(n, 77 + k)
+    got, _ = gpu_ntt(libs, stream, data, orc.omega(k), rotate=rot)
+    assert np.array_equal(got, orc.ntt(data, k, THREADS))
+
+
+def test_round_trip_2_16(libs, stream):
+    """Config 1: NTT then INTT (omega^-1 table, then n^-1 through distribute_powers)."""
+    k, n = 16, 1 << 16
+    data = random_elements(n, 2016)
+    buf, x = upload(data)
+    tw_f_buf, tw_f = upload(twiddle_table(libs, orc.omega(k), n))
+    tw_i_buf, tw_i = upload(twiddle_table(libs, orc.omega(k, True), n))
+    ninv_buf, ninv = upload(mont_limbs(pow(n, -1, FR_MODULUS)))
+    ntt, dist = SsipNtt(libs).get_fn().f, DistributePowers(libs).get_fn().f
+    ntt([x], [tw_f, stream])
+    stream.sync()
+    fwd = download(x, buf)
+    assert np.array_equal(fwd, orc.ntt(data, k, THREADS))
+    ntt([x], [tw_i, stream])
+    dist([x], [ninv, stream])
+    stream.sync()
+    assert np.array_equal(download(x, buf), data)
+
+
+def test_distribute_zeta_extended(libs, stream):
+    """core/tests/ntt.rs:261-388: pad to 4n, multiply by [1, zeta, zeta^2] cyclically, NTT."""
+    k = 12
+    ek = k + 2
+    data = random_elements(1 << k, 31)
+    padded = np.zeros((1 << ek, 4), dtype=np.uint64)
+    padded[: 1 << k] = data
+    zeta = pow(7, (FR_MODULUS - 1) // 3, FR_MODULUS)
+    zetas = np.stack([mont_limbs(1), mont_limbs(zeta), mont_limbs(zeta * zeta % FR_MODULUS)])
+    buf, x = device_array(1 << ek)
+    x.rotate(-2)
+    cpu2gpu(padded, x, buf)
+    z_buf, z = upload(zetas)
+    tw_buf, tw = upload(twiddle_table(libs, orc.omega(ek), 1 << ek))
+    DistributePowers(libs).get_fn().f([x], [z, stream])
+    SsipNtt(libs).get_fn().f([x], [tw, stream])
+    stream.sync()
+    want = orc.ntt(orc.distribute_powers(padded, zetas), ek, THREADS)
+    assert np.array_equal(gpu2cpu(x, buf), want)
+
+
+def _device_coset_powers(libs, stream, n, g):
+    pw_buf, pw = device_array(n)
+    base_buf, _ = upload(mont_limbs(g))
+    gen_powers(libs, pw.values, n, base_buf.addr, stream)
+    stream.sync()
+    return pw_buf, pw
+
+
+def test_gen_powers_matches_oracle(libs, stream):
+    n = 5000
+    pw_buf, pw = _device_coset_powers(libs, stream, n, 7)
+    got = download(pw, pw_buf)
+    assert np.array_equal(got, orc.twiddles(n, to_mont(7)))
+
+
+@pytest.mark.parametrize("k", [20, 22, 24])
+def test_workload_sizes_forward_inverse_coset(libs, stream, k):
+    """Config 2 sizes: forward, inverse (with n^-1) and coset g = 7, bit-exact."""
+    n = 1 << k
+    data = random_elements(n, 2000 + k)
+    buf, x = upload(data)
+    ntt, dist = SsipNtt(libs).get_fn().f, DistributePowers(libs).get_fn().f
+    tw_buf, tw = upload(twiddle_table(libs, orc.omega(k), n))
+    ntt([x], [tw, stream])
+    stream.sync()
+    fwd = download(x, buf)
+    assert np.array_equal(fwd, orc.ntt(data, k, THREADS)), "forward"
+    del tw_buf
+    twi_buf, twi = upload(twiddle_table(libs, orc.omega(k, True), n))
+    ninv_buf, ninv = upload(mont_limbs(pow(n, -1, FR_MODULUS)))
+    ntt([x], [twi, stream])
+    dist([x], [ninv, stream])
+    stream.sync()
+    assert np.array_equal(download(x, buf), data), "inverse round trip"
+    del twi_buf
+    # coset: NTT(x_i g^i)
+    pw_buf, pw = _device_coset_powers(libs, stream, n, 7)
+    tw_buf, tw = upload(twiddle_table(libs, orc.omega(k), n))
+    dist([x], [pw, stream])
+    ntt([x], [tw, stream])
+    stream.sync()
+    want = orc.ntt(orc.distribute_powers(data, orc.twiddles(n, to_mont(7))), k, THREADS)
+    assert np.array_equal(download(x, buf), want), "coset"
+
+
+def test_extended_coset_4x_of_2_22(libs, stream):
+    """Config 2: 2^22 coefficients zero-padded to 2^24, coset g = 7, NTT with omega_{2^24}."""
+    k, ek = 22, 24
+    data = random_elements(1 << k, 4422)
+    padded = np.zeros((1 << ek, 4), dtype=np.uint64)
+    padded[: 1 << k] = data
+    buf, x = upload(padded)
+    pw_buf, pw = _device_coset_powers(libs, stream, 1 << ek, 7)
+    tw_buf, tw = upload(twiddle_table(libs, orc.omega(ek), 1 << ek))
+    DistributePowers(libs).get_fn().f([x], [pw, stream])
+    SsipNtt(libs).get_fn().f([x], [tw, stream])
+    stream.sync()
+    want = orc.ntt(orc.distribute_powers(padded, orc.twiddles(1 << ek, to_mont(7))), ek, THREADS)
+    assert np.array_equal(download(x, buf), want)

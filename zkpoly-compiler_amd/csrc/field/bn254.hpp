@@ -1,0 +1,318 @@
+// BN254 prime fields (Fr scalar field, Fq base field) for gfx950 and host.
+//
+// Representation: 8 x u32 little-endian limbs holding the canonical Montgomery
+// form x*R mod p (R = 2^256), always fully reduced (< p).  This is byte-identical
+// to halo2curves' 4 x u64 Montgomery limbs, which is what the reference runtime
+// hands to its kernels (core/src/ntt.rs:186, core/src/msm.rs:325; limb layout
+// core/src/common/mont/src/field.cuh:637-644).
+//
+// Multiplication is CIOS Montgomery on 32-bit limbs, written so that every
+// limb product is one 32x32+64 -> 64 multiply-add (v_mad_u64_u32 on CDNA).
+// The "no final carry word" form is valid because the top limb of both moduli
+// is below 2^31 - 1, so the running value never exceeds 2p < 2^255.
+// Semantics follow the reference's mont::Element (field.cuh:907-1204):
+// canonical output, zero-preserving negation, Fermat inversion.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ZKP_HD __host__ __device__ __forceinline__
+
+namespace zkp {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+struct FrParams {
+  // r = 0x30644e72e131a029b85045b68181585d2833e84879b9709143e1f593f0000001
+  static constexpr u32 P[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr u32 INV = 0xefffffffu;  // -r^-1 mod 2^32
+  static constexpr u32 ONE[8] = {0x4ffffffbu, 0xac96341cu, 0x9f60cd29u, 0x36fc7695u,
+                                 0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr u32 R2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
+                                0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+};
+
+struct FqParams {
+  // q = 0x30644e72e131a029b85045b68181585d97816a916871ca8d3c208c16d87cfd47
+  static constexpr u32 P[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                               0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  static constexpr u32 INV = 0xe4866389u;  // -q^-1 mod 2^32
+  static constexpr u32 ONE[8] = {0xc58f0d9du, 0xd35d438du, 0xf5c70b3du, 0x0a78eb28u,
+                                 0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
+  static constexpr u32 R2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
+                                0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+};
+
+template <class Params>
+struct Fp {
+  static constexpr int LIMBS = 8;
+  u32 v[8];
+
+  ZKP_HD static Fp zero() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = 0;
+    return r;
+  }
+  ZKP_HD static Fp one() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = Params::ONE[i];
+    return r;
+  }
+  ZKP_HD static Fp from_raw(const u32* p) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = p[i];
+    return r;
+  }
+
+  // 2 x 16-byte vector loads / stores (elements are 32-byte aligned in HBM).
+  __device__ __forceinline__ static Fp load(const u32* p) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    uint4 a = q[0], b = q[1];
+    Fp r;
+    r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+    r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+    return r;
+  }
+  __device__ __forceinline__ void store(u32* p) const {
+    uint4* q = reinterpret_cast<uint4*>(p);
+    q[0] = make_uint4(v[0], v[1], v[2], v[3]);
+    q[1] = make_uint4(v[4], v[5], v[6], v[7]);
+  }
+
+  ZKP_HD bool is_zero() const {
+    u32 acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc |= v[i];
+    return acc == 0;
+  }
+  ZKP_HD bool operator==(const Fp& o) const {
+    u32 acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc |= v[i] ^ o.v[i];
+    return acc == 0;
+  }
+  ZKP_HD bool operator!=(const Fp& o) const { return !(*this == o); }
+
+  // r = x - p if x >= p else x (x < 2p).
+  ZKP_HD static Fp reduce_once(const Fp& x) {
+    Fp d;
+    u64 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      u64 t = (u64)x.v[i] - Params::P[i] - borrow;
+      d.v[i] = (u32)t;
+      borrow = (t >> 32) & 1;
+    }
+    // borrow set <=> x < p
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = borrow ? x.v[i] : d.v[i];
+    return r;
+  }
+
+  ZKP_HD Fp operator+(const Fp& o) const {
+    Fp s;
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      c = (u64)v[i] + o.v[i] + (c >> 32);
+      s.v[i] = (u32)c;
+    }
+    // 2p < 2^255: no carry out of limb 7
+    return reduce_once(s);
+  }
+
+  ZKP_HD Fp operator-(const Fp& o) const {
+    Fp d;
+    u64 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      u64 t = (u64)v[i] - o.v[i] - borrow;
+      d.v[i] = (u32)t;
+      borrow = (t >> 32) & 1;
+    }
+    // add p back if the subtraction wrapped
+    u32 mask = 0u - (u32)borrow;
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      c = (u64)d.v[i] + (Params::P[i] & mask) + (c >> 32);
+      d.v[i] = (u32)c;
+    }
+    return d;
+  }
+
+  ZKP_HD Fp neg() const {
+    // zero stays zero (reference field.cuh:1066-1079)
+    Fp d;
+    u64 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      u64 t = (u64)Params::P[i] - v[i] - borrow;
+      d.v[i] = (u32)t;
+      borrow = (t >> 32) & 1;
+    }
+    u32 nz = is_zero() ? 0u : 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < 8; i++) d.v[i] &= nz;
+    return d;
+  }
+
+  ZKP_HD Fp dbl() const { return *this + *this; }
+
+  // Montgomery product a*b*R^-1 mod p.
+  //
+  // Device: product scanning (column-wise) with interleaved reduction.  Each
+  // 32x32 limb product is one v_mad_u64_u32 into a 64-bit column accumulator
+  // whose carry-out (VCC) is folded into a third word by one v_addc/v_cndmask,
+  // i.e. 2 VALU ops per limb product and no register-pair shuffling (128 limb
+  // products + 8 m-digit multiplies per call).
+  // Host: CIOS with the same interleaved reduction (identical results; both
+  // return the canonical representative).
+  ZKP_HD Fp operator*(const Fp& b) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return mul_ps(*this, b);
+#else
+    return mul_cios(*this, b);
+#endif
+  }
+
+  ZKP_HD static Fp mul_cios(const Fp& a, const Fp& b) {
+    u32 t[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) t[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const u32 bi = b.v[i];
+      u64 A = (u64)a.v[0] * bi + t[0];
+      const u32 t0 = (u32)A;
+      const u32 m = t0 * Params::INV;
+      u64 C = (u64)m * Params::P[0] + t0;
+#pragma unroll
+      for (int j = 1; j < 8; j++) {
+        A = (u64)a.v[j] * bi + t[j] + (A >> 32);
+        C = (u64)m * Params::P[j] + (u32)A + (C >> 32);
+        t[j - 1] = (u32)C;
+      }
+      t[7] = (u32)(C >> 32) + (u32)(A >> 32);
+    }
+    return reduce_once(from_raw(t));
+  }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+  // acc(64) += a*b; c2 += carry-out of the 64-bit accumulate.
+  __device__ __forceinline__ static void mac(u64& acc, u32& c2, u32 a, u32 b) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(c2)
+        : "v"(a), "v"(b)
+        : "vcc");
+  }
+  // first product of a column: c2 = carry-out (starts the column's top word)
+  __device__ __forceinline__ static void mac_first(u64& acc, u32& c2, u32 a, u32 b) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_cndmask_b32_e64 %1, 0, 1, vcc"
+        : "+v"(acc), "=v"(c2)
+        : "v"(a), "v"(b)
+        : "vcc");
+  }
+  __device__ __forceinline__ static Fp mul_ps(const Fp& A, const Fp& B) {
+    const u32* a = A.v;
+    const u32* b = B.v;
+    u32 m[8];
+    u32 t[8];
+    u64 acc = 0;
+    u32 c2 = 0;
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int j = k - i;
+        if (j < 0 || j > 7) continue;
+        if (first) {
+          mac_first(acc, c2, a[i], b[j]);
+          first = false;
+        } else {
+          mac(acc, c2, a[i], b[j]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int j = k - i;
+        if (i >= k || j < 1 || j > 7) continue;
+        mac(acc, c2, m[i], Params::P[j]);
+      }
+      if (k < 8) {
+        m[k] = (u32)acc * Params::INV;
+        mac(acc, c2, m[k], Params::P[0]);
+      } else {
+        t[k - 8] = (u32)acc;
+      }
+      acc = (acc >> 32) | ((u64)c2 << 32);
+    }
+    t[7] = (u32)acc;
+    return reduce_once(from_raw(t));
+  }
+#endif
+
+  ZKP_HD Fp sqr() const { return (*this) * (*this); }
+
+  // this^e for a 64-bit exponent (square-and-multiply, MSB first).
+  ZKP_HD Fp pow(u64 e) const {
+    Fp r = one();
+    bool started = false;
+    for (int bit = 63; bit >= 0; bit--) {
+      if (started) r = r.sqr();
+      if ((e >> bit) & 1) {
+        r = started ? r * (*this) : *this;
+        started = true;
+      }
+    }
+    return r;
+  }
+
+  // this^e for an exponent of nwords little-endian u32 words.
+  ZKP_HD Fp pow_words(const u32* e, int nwords) const {
+    Fp r = one();
+    for (int w = nwords - 1; w >= 0; w--) {
+      for (int bit = 31; bit >= 0; bit--) {
+        r = r.sqr();
+        if ((e[w] >> bit) & 1) r = r * (*this);
+      }
+    }
+    return r;
+  }
+
+  // Fermat inverse x^(p-2); inverse of zero is zero.
+  ZKP_HD Fp inv() const {
+    u32 e[8];
+    u64 borrow = 2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      u64 t = (u64)Params::P[i] - borrow;
+      e[i] = (u32)t;
+      borrow = (t >> 63) & 1;
+    }
+    return pow_words(e, 8);
+  }
+
+  // canonical integer -> Montgomery form (x * R^2 * R^-1)
+  ZKP_HD static Fp from_canonical(const Fp& x) { return x * from_raw(Params::R2); }
+  // Montgomery form -> canonical integer (x * 1 * R^-1)
+  ZKP_HD Fp to_canonical() const {
+    Fp o = zero();
+    o.v[0] = 1;
+    return (*this) * o;
+  }
+};
+
+typedef Fp<FrParams> Fr;
+typedef Fp<FqParams> Fq;
+
+}  // namespace zkp

@@ -1,0 +1,314 @@
+// libntt.so -- BN254 Fr number-theoretic transforms for gfx950.
+//
+// C ABI identical to the reference's core/src/ntt/src/ntt.h:6-17 (cudaError_t
+// -> hipError_t, cudaStream_t -> hipStream_t), so the reference runtime's
+// closures (core/src/ntt.rs:165-200, 112-150) bind these symbols unchanged.
+//
+// Transform: in-place forward DFT X[j] = sum_i a[i] w^(ij) with natural order
+// in and out (matches halo2's best_fft, core/tests/ntt.rs:58,139-143).  The
+// network is exactly best_fft's radix-2 decimation-in-time network
+// (core/tests/msm_halo2.rs:221-324): a bit-reversal permutation followed by
+// log2(n) butterfly stages b' = a - b*w, a' = a + b*w with stage-s twiddle
+// w = tw[(i mod 2^s) * n / 2^(s+1)].  Because the butterflies and twiddles are
+// the same, the output equals best_fft even for an omega that is not a root of
+// unity (the reference's own tests use a random omega, core/tests/ntt.rs:53).
+//
+// HBM layout: the polynomial is the caller's array of 32-byte elements (8 x u32
+// Montgomery limbs), addressed through the PolyPtr index map; the twiddle table
+// is the caller's [1, w, ..., w^(n/2-1)] (what ssip_precompute writes, the same
+// table the reference's compiler stores as a constant).
+//
+// Kernels:
+//  * bitrev_kernel: in-place swap of i <-> rev(i).
+//  * dit_pass<TILE_LOG>: a workgroup loads a tile of 2^TILE_LOG elements --
+//    2^d butterfly-connected rows times 2^c adjacent columns, so every global
+//    access is a run of >= 4 consecutive 32-byte elements -- into LDS (two
+//    16-byte planes), runs d consecutive stages there, and writes it back.
+//    A transform of 2^24 is bitrev + 3 passes (11 + 9 + 4 stages).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../common/poly_ptr.hpp"
+#include "../field/bn254.hpp"
+
+namespace zkp {
+namespace ntt {
+
+constexpr int THREADS = 256;
+constexpr int TILE_LOG = 11;          // 2048 elements = 64 KiB of LDS per workgroup
+constexpr int MIN_COL_LOG_LATER = 2;  // later passes read runs of >= 4 elements (128 B)
+
+__device__ __forceinline__ Fr load_elem(const PolyView& v, uint64_t i) { return Fr::load(v.elem(i)); }
+__device__ __forceinline__ void store_elem(const PolyView& v, uint64_t i, const Fr& x) { x.store(v.elem(i)); }
+
+__global__ void __launch_bounds__(THREADS) bitrev_kernel(PolyView v, uint32_t log_n) {
+  const uint64_t n = 1ull << log_n;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t j = __brevll(i) >> (64 - log_n);
+    if (i < j) {
+      Fr a = load_elem(v, i);
+      Fr b = load_elem(v, j);
+      store_elem(v, i, b);
+      store_elem(v, j, a);
+    }
+  }
+}
+
+// One pass of DIT stages [s0, s0 + d) over tiles of 2^d rows x 2^c_log columns.
+//   s0 > 0 : columns are consecutive low indices, rows stride 2^s0
+//            idx(c, k) = lo_block * C + c + (k << s0) + (hi << (s0 + d))
+//   s0 == 0: columns are consecutive contiguous groups of 2^d elements
+//            idx(c, k) = (tile * C + c) << d | k
+__global__ void __launch_bounds__(THREADS)
+    dit_pass(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+             uint32_t c_log) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+
+  const uint32_t C = 1u << c_log;
+  const uint32_t R = 1u << d;
+  const uint32_t T = R << c_log;
+  const uint64_t tile = blockIdx.x;
+
+  uint64_t base, cstride, kstride;
+  if (s0 == 0) {
+    base = (tile << c_log) << d;
+    cstride = R;
+    kstride = 1;
+  } else {
+    const uint64_t lo_blocks = (1ull << s0) >> c_log;
+    const uint64_t hi = tile / lo_blocks;
+    const uint64_t lo_block = tile % lo_blocks;
+    base = lo_block * C + (hi << (s0 + d));
+    cstride = 1;
+    kstride = 1ull << s0;
+  }
+
+  // load: consecutive threads take consecutive global addresses
+  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    uint32_t c, k;
+    if (cstride == 1) {
+      c = e & (C - 1);
+      k = e >> c_log;
+    } else {
+      k = e & (R - 1);
+      c = e >> d;
+    }
+    const uint64_t idx = base + c * cstride + (uint64_t)k * kstride;
+    const uint4* src = reinterpret_cast<const uint4*>(v.elem(idx));
+    const uint32_t pos = (k << c_log) | c;
+    lds0[pos] = src[0];
+    lds1[pos] = src[1];
+  }
+  __syncthreads();
+
+  for (uint32_t m = 0; m < d; m++) {
+    const uint32_t s = s0 + m;
+    const uint32_t half = 1u << m;
+    const uint64_t jmask = (1ull << s) - 1;
+    const uint32_t tw_shift = log_n - 1 - s;
+    for (uint32_t b = threadIdx.x; b < (T >> 1); b += THREADS) {
+      const uint32_t c = b & (C - 1);
+      const uint32_t q = b >> c_log;
+      const uint32_t ka = ((q >> m) << (m + 1)) | (q & (half - 1));
+      const uint32_t kb = ka + half;
+      const uint32_t pa = (ka << c_log) | c;
+      const uint32_t pb = (kb << c_log) | c;
+      const uint64_t idx_a = base + c * cstride + (uint64_t)ka * kstride;
+      const uint64_t j = idx_a & jmask;
+      const Fr w = Fr::load(tw + ((j << tw_shift) << 3));
+      Fr x, y;
+      {
+        uint4 a0 = lds0[pa], a1 = lds1[pa], b0 = lds0[pb], b1 = lds1[pb];
+        x.v[0] = a0.x; x.v[1] = a0.y; x.v[2] = a0.z; x.v[3] = a0.w;
+        x.v[4] = a1.x; x.v[5] = a1.y; x.v[6] = a1.z; x.v[7] = a1.w;
+        y.v[0] = b0.x; y.v[1] = b0.y; y.v[2] = b0.z; y.v[3] = b0.w;
+        y.v[4] = b1.x; y.v[5] = b1.y; y.v[6] = b1.z; y.v[7] = b1.w;
+      }
+      const Fr t = y * w;
+      const Fr u = x + t;
+      const Fr l = x - t;
+      lds0[pa] = make_uint4(u.v[0], u.v[1], u.v[2], u.v[3]);
+      lds1[pa] = make_uint4(u.v[4], u.v[5], u.v[6], u.v[7]);
+      lds0[pb] = make_uint4(l.v[0], l.v[1], l.v[2], l.v[3]);
+      lds1[pb] = make_uint4(l.v[4], l.v[5], l.v[6], l.v[7]);
+    }
+    __syncthreads();
+  }
+
+  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    uint32_t c, k;
+    if (cstride == 1) {
+      c = e & (C - 1);
+      k = e >> c_log;
+    } else {
+      k = e & (R - 1);
+      c = e >> d;
+    }
+    const uint64_t idx = base + c * cstride + (uint64_t)k * kstride;
+    uint4* dst = reinterpret_cast<uint4*>(v.elem(idx));
+    const uint32_t pos = (k << c_log) | c;
+    dst[0] = lds0[pos];
+    dst[1] = lds1[pos];
+  }
+}
+
+// out[i] = base^i for i < len; each thread starts its chunk with one pow.
+__global__ void __launch_bounds__(THREADS)
+    gen_powers_kernel(uint32_t* __restrict__ out, uint64_t len, const uint32_t* __restrict__ base_ptr,
+                      uint32_t chunk) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t start = t * chunk;
+  if (start >= len) return;
+  const Fr base = Fr::load(base_ptr);
+  Fr x = base.pow(start);
+  const uint64_t end = start + chunk < len ? start + chunk : len;
+  for (uint64_t i = start; i < end; i++) {
+    x.store(out + i * 8);
+    x = x * base;
+  }
+}
+
+// x[i] *= powers[i mod m]  (core/src/ntt/src/distribute_powers.cuh:4-19)
+__global__ void __launch_bounds__(THREADS)
+    distribute_powers_kernel(PolyView v, const uint32_t* __restrict__ powers, uint64_t m, uint64_t mask,
+                             bool pow2) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < v.len; i += stride) {
+    const uint64_t pi = pow2 ? (i & mask) : (i % m);
+    const Fr p = Fr::load(powers + pi * 8);
+    store_elem(v, i, load_elem(v, i) * p);
+  }
+}
+
+struct PassPlan {
+  uint32_t s0, d, c_log;
+  uint64_t tiles;
+};
+
+// Splits log_n DIT stages into LDS passes.  Pass 0 takes up to TILE_LOG stages
+// over contiguous groups; later passes take <= TILE_LOG - MIN_COL_LOG_LATER
+// stages each, balanced, with as many adjacent columns as the tile holds.
+static int plan_passes(uint32_t log_n, PassPlan* out) {
+  int np = 0;
+  uint32_t d0 = log_n < (uint32_t)TILE_LOG ? log_n : (uint32_t)TILE_LOG;
+  uint32_t c0 = TILE_LOG - d0;
+  if (c0 > log_n - d0) c0 = log_n - d0;
+  out[np++] = PassPlan{0, d0, c0, (1ull << log_n) >> (d0 + c0)};
+  uint32_t s0 = d0;
+  const uint32_t max_d = TILE_LOG - MIN_COL_LOG_LATER;
+  while (s0 < log_n) {
+    const uint32_t rem = log_n - s0;
+    const uint32_t parts = (rem + max_d - 1) / max_d;
+    const uint32_t d = (rem + parts - 1) / parts;
+    uint32_t c = TILE_LOG - d;
+    if (c > s0) c = s0;
+    out[np++] = PassPlan{s0, d, c, (1ull << log_n) >> (d + c)};
+    s0 += d;
+  }
+  return np;
+}
+
+static inline unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
+  uint64_t g = (work + threads - 1) / threads;
+  if (g > cap) g = cap;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace ntt
+}  // namespace zkp
+
+using namespace zkp;
+using namespace zkp::ntt;
+
+extern "C" {
+
+// In-place natural-order forward NTT of x (len = 2^log_len) with twiddle table
+// [1, w, ..., w^(len/2-1)] on the device.  Replaces ssip_ntt of
+// core/src/ntt/src/ntt.h:6 / ntt.cu:7-9.  The two thread-shape hints tune the
+// reference's CUDA kernels and are accepted but not needed here.
+hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
+                    const unsigned int max_threads_stage1_log, const unsigned int max_threads_stage2_log) {
+  (void)max_threads_stage1_log;
+  (void)max_threads_stage2_log;
+  PolyView v;
+  if (!make_view(x, &v)) return hipErrorInvalidValue;
+  if (log_len > 40 || v.len != (1ull << log_len)) return hipErrorInvalidValue;
+  if (log_len == 0) return hipSuccess;
+  if (v.ptr == nullptr || twiddle == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
+                     (uint32_t)log_len);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  PassPlan plan[48];
+  const int np = plan_passes(log_len, plan);
+  for (int p = 0; p < np; p++) {
+    hipLaunchKernelGGL(dit_pass, dim3((unsigned)plan[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
+                       (uint32_t)log_len, plan[p].s0, plan[p].d, plan[p].c_log);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+// twiddle[i] = unit^i for i < len, written to HOST memory; synchronous.
+// Replaces ssip_precompute of core/src/ntt/src/ntt.h:9 / ntt.cu:11-14 (the
+// reference computes the same prefix product with a device scan,
+// precompute.cuh:8-86).  The second argument is the table length (the Rust
+// wrapper passes twiddle.len, core/src/ntt.rs:221-226).
+hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len, const unsigned int* unit) {
+  if (len == 0) return hipSuccess;
+  if (twiddle == nullptr || unit == nullptr) return hipErrorInvalidValue;
+  uint32_t *d_out = nullptr, *d_unit = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (err != hipSuccess) return err;
+  err = hipMalloc(&d_out, (size_t)len * 32);
+  if (err == hipSuccess) err = hipMalloc(&d_unit, 32);
+  if (err == hipSuccess) err = hipMemcpyAsync(d_unit, unit, 32, hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) {
+    const uint32_t chunk = 256;
+    const uint64_t threads = ((uint64_t)len + chunk - 1) / chunk;
+    hipLaunchKernelGGL(gen_powers_kernel, dim3(grid_for(threads, THREADS, 1u << 30)), dim3(THREADS), 0, s,
+                       d_out, (uint64_t)len, d_unit, chunk);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpyAsync(twiddle, d_out, (size_t)len * 32, hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (d_out) hipFree(d_out);
+  if (d_unit) hipFree(d_unit);
+  hipStreamDestroy(s);
+  return err;
+}
+
+// x[i] *= powers[i mod power_num] on the device.  Replaces distribute_powers of
+// core/src/ntt/src/ntt.h:17 / distribute_powers.cuh:13-19.
+hipError_t distribute_powers(PolyPtr poly, const unsigned int* powers, unsigned long long power_num,
+                             hipStream_t stream) {
+  PolyView v;
+  if (!make_view(poly, &v)) return hipErrorInvalidValue;
+  if (v.len == 0) return hipSuccess;
+  if (v.ptr == nullptr || powers == nullptr || power_num == 0) return hipErrorInvalidValue;
+  const bool pow2 = (power_num & (power_num - 1)) == 0;
+  hipLaunchKernelGGL(distribute_powers_kernel, dim3(grid_for(v.len, THREADS, 8192)), dim3(THREADS), 0,
+                     stream, v, powers, (uint64_t)power_num, (uint64_t)(power_num - 1), pow2);
+  return hipGetLastError();
+}
+
+// out[i] = base^i for i < len, device memory, asynchronous on stream.  No
+// counterpart in the reference ABI: used to build coset power tables and
+// twiddle tables directly in HBM.
+hipError_t zkp_gen_powers(unsigned int* out, unsigned long long len, const unsigned int* base,
+                          hipStream_t stream) {
+  if (len == 0) return hipSuccess;
+  if (out == nullptr || base == nullptr) return hipErrorInvalidValue;
+  const uint32_t chunk = 256;
+  const uint64_t threads = (len + chunk - 1) / chunk;
+  hipLaunchKernelGGL(gen_powers_kernel, dim3(grid_for(threads, THREADS, 1u << 30)), dim3(THREADS), 0, stream,
+                     out, (uint64_t)len, base, chunk);
+  return hipGetLastError();
+}
+
+}  // extern "C"

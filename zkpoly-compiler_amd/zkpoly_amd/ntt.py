@@ -1,0 +1,115 @@
+"""NTT operators over libntt.so, mirroring core/src/ntt.rs.
+
+SsipNtt (core/src/ntt.rs:165-200), SsipPrecompute (:204-238) and
+DistributePowers (:112-150) with the same argument lists and checks:
+  SsipNtt:          mut_vars=[poly], vars=[twiddle (len n/2), stream]
+  DistributePowers: mut_vars=[poly], vars=[powers, stream]
+  SsipPrecompute:   f(twiddle_host_array, unit) fills [1, w, ..., w^(len-1)]
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import hip
+from .runtime import (
+    FuncMeta,
+    Function,
+    KernelType,
+    Libs,
+    PolyPtr,
+    RegisteredFunction,
+    ScalarArray,
+    unwrap_scalar_array,
+    unwrap_stream,
+)
+
+LIB_NAME = "libntt.so"
+
+# (max_threads_stage1_log, max_threads_stage2_log) for 32-byte fields
+# (core/src/ntt.rs:373-378); forwarded for ABI compatibility.
+STAGE_THREADS = (8, 8)
+
+
+def load_lib(libs: Libs) -> ctypes.CDLL:
+    lib = libs.load_relative(LIB_NAME)
+    if not getattr(lib, "_zkp_typed", False):
+        u32p = ctypes.c_void_p
+        lib.ssip_ntt.argtypes = [PolyPtr, u32p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint]
+        lib.ssip_ntt.restype = ctypes.c_int
+        lib.ssip_precompute.argtypes = [u32p, ctypes.c_uint, u32p]
+        lib.ssip_precompute.restype = ctypes.c_int
+        lib.distribute_powers.argtypes = [PolyPtr, u32p, ctypes.c_ulonglong, ctypes.c_void_p]
+        lib.distribute_powers.restype = ctypes.c_int
+        lib.zkp_gen_powers.argtypes = [u32p, ctypes.c_ulonglong, u32p, ctypes.c_void_p]
+        lib.zkp_gen_powers.restype = ctypes.c_int
+        lib._zkp_typed = True
+    return lib
+
+
+class SsipNtt(RegisteredFunction):
+    def __init__(self, libs: Libs):
+        self.c_func = load_lib(libs).ssip_ntt
+
+    def get_fn(self) -> Function:
+        c_func = self.c_func
+
+        def f(mut_var, var, gpu_mapping=None):
+            assert len(mut_var) == 1
+            assert len(var) == 2
+            x = unwrap_scalar_array(mut_var[0])
+            assert x.len > 0 and (x.len & (x.len - 1)) == 0, "len must be a power of two"
+            log_len = x.len.bit_length() - 1
+            twiddle = unwrap_scalar_array(var[0])
+            assert twiddle.len * 2 == x.len
+            stream = unwrap_stream(var[1])
+            hip.set_device(stream.get_device())
+            hip.check(
+                c_func(PolyPtr.from_array(x), twiddle.values, log_len, stream.raw(), *STAGE_THREADS),
+                "ssip_ntt",
+            )
+
+        return Function(FuncMeta("ssip_ntt", KernelType.NttPrcompute), f)
+
+
+class DistributePowers(RegisteredFunction):
+    def __init__(self, libs: Libs):
+        self.c_func = load_lib(libs).distribute_powers
+
+    def get_fn(self) -> Function:
+        c_func = self.c_func
+
+        def f(mut_var, var, gpu_mapping=None):
+            assert len(mut_var) == 1
+            assert len(var) == 2
+            x = unwrap_scalar_array(mut_var[0])
+            powers = unwrap_scalar_array(var[0])
+            stream = unwrap_stream(var[1])
+            hip.set_device(stream.get_device())
+            hip.check(c_func(PolyPtr.from_array(x), powers.values, powers.len, stream.raw()), "distribute_powers")
+
+        return Function(FuncMeta("distribute_powers", KernelType.DistributePowers), f)
+
+
+class SsipPrecompute:
+    """Compile-time twiddle generation (not a RegisteredFunction in the reference either)."""
+
+    def __init__(self, libs: Libs):
+        self.c_func = load_lib(libs).ssip_precompute
+
+    def get_fn(self):
+        c_func = self.c_func
+
+        def f(twiddle: ScalarArray, unit: np.ndarray):
+            assert twiddle.len > 0 and (twiddle.len & (twiddle.len - 1)) == 0
+            u = np.ascontiguousarray(unit, dtype=np.uint64)
+            assert u.size == 4
+            hip.check(c_func(twiddle.values, twiddle.len, u.ctypes.data), "ssip_precompute")
+
+        return f
+
+
+def gen_powers(libs: Libs, out_dev: int, length: int, base_dev: int, stream: hip.Stream) -> None:
+    """[1, b, b^2, ...] written to device memory (no reference counterpart)."""
+    hip.check(load_lib(libs).zkp_gen_powers(out_dev, length, base_dev, stream.raw()), "zkp_gen_powers")
