@@ -1,0 +1,303 @@
+#!/usr/bin/env python3
+"""Benchmark: BN254 NTT 2^24 Gelem/s on one MI355X (plus the NTT suite, MSM
+2^20 and the batched form when built), bit-exact against the CPU oracle.
+
+A step is one in-place forward NTT of 2^24 BN254 Fr elements (synthetic,
+uniform, resident in HBM) through the reference's ssip_ntt C ABI.  The timed
+region is K steps between two events on the launch stream; per-kernel times
+come from events recorded between the kernels of those same launches.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "zkpoly-compiler_amd")
+for _p in (PKG, ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# 256 CUs x 64 lanes x 2.4 GHz: one v_mad_u64_u32 per lane per clock
+INT_MUL_PEAK = 256 * 64 * 2.4e9
+MADS_PER_MODMUL = 136
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--quick", action="store_true", help="headline NTT only (no suite / MSM / batched)")
+    return ap.parse_args()
+
+
+class Ntt:
+    """Device-resident NTT workload of size 2^k through libntt.so."""
+
+    def __init__(self, libs, stream, k, seed):
+        from zkpoly_amd import hip
+        from zkpoly_amd.field import mont_limbs, omega, random_elements
+        from zkpoly_amd.ntt import gen_powers, load_lib
+        from zkpoly_amd.runtime import PolyPtr, ScalarArray
+
+        self.k, self.n = k, 1 << k
+        self.lib = load_lib(libs)
+        self.stream = stream
+        self.host = random_elements(self.n, seed)
+        self.buf = hip.DeviceBuffer(self.n * 32)
+        self.buf.upload(self.host)
+        self.x = ScalarArray(self.n, self.buf.addr)
+        self.ptr = PolyPtr.from_array(self.x)
+        self.tw = {}
+        for inv in (False, True):
+            wb = hip.DeviceBuffer(32)
+            wb.upload(mont_limbs(omega(k, inv)))
+            t = hip.DeviceBuffer(self.n // 2 * 32)
+            gen_powers(libs, t.addr, self.n // 2, wb.addr, stream)
+            self.tw[inv] = t
+        nb = hip.DeviceBuffer(32)
+        from zkpoly_amd.field import FR_MODULUS
+
+        nb.upload(mont_limbs(pow(self.n, -1, FR_MODULUS)))
+        self.ninv = nb
+        stream.sync()
+
+    def reset(self):
+        self.buf.upload(self.host)
+
+    def ntt(self, inverse=False):
+        rc = self.lib.ssip_ntt(self.ptr, self.tw[inverse].addr, self.k, self.stream.raw(), 8, 8)
+        if rc:
+            raise RuntimeError(f"ssip_ntt rc={rc}")
+        if inverse:
+            rc = self.lib.distribute_powers(self.ptr, self.ninv.addr, 1, self.stream.raw())
+            if rc:
+                raise RuntimeError(f"distribute_powers rc={rc}")
+
+    def download(self):
+        return self.buf.download(np.uint64, self.n * 4).reshape(self.n, 4)
+
+
+def time_loop(stream, fn, steps, warmup):
+    from zkpoly_amd import hip
+
+    for _ in range(warmup):
+        fn()
+    stream.sync()
+    a, b = hip.Event(), hip.Event()
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    b.sync()
+    return a.elapsed_ms(b) / steps
+
+
+def headline(libs, stream, args):
+    """K timed forward NTTs of 2^log_n with per-kernel events."""
+    from zkpoly_amd import hip
+    from zkpoly_amd.runtime import PolyPtr  # noqa: F401
+
+    w = Ntt(libs, stream, args.log_n, seed=2024)
+    lib = w.lib
+    lib.zkp_ntt_with_events.argtypes = [
+        type(w.ptr),
+        ctypes.c_void_p,
+        ctypes.c_uint,
+        ctypes.c_void_p,
+        ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_int,
+        ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.c_int),
+    ]
+    lib.zkp_ntt_with_events.restype = ctypes.c_int
+    max_ev = 16
+    # bit-exact check of one step against the oracle (and CPU baseline timing)
+    w.ntt()
+    stream.sync()
+    gpu_out = w.download()
+    w.reset()
+    for _ in range(args.warmup):
+        w.ntt()
+    stream.sync()
+    events = [[hip.Event() for _ in range(max_ev)] for _ in range(args.steps)]
+    ev_arrays = [(ctypes.c_void_p * max_ev)(*[e.handle.value for e in evs]) for evs in events]
+    kinds = (ctypes.c_int * max_ev)()
+    nk = ctypes.c_int(0)
+    t0, t1 = hip.Event(), hip.Event()
+    t0.record(stream)
+    for s in range(args.steps):
+        rc = lib.zkp_ntt_with_events(w.ptr, w.tw[False].addr, w.k, stream.raw(), ev_arrays[s], max_ev, kinds, ctypes.byref(nk))
+        if rc:
+            raise RuntimeError(f"zkp_ntt_with_events rc={rc}")
+    t1.record(stream)
+    t1.sync()
+    total_ms = t0.elapsed_ms(t1)
+    nkern = nk.value
+    per_kind = {0: [], 1: []}
+    for s in range(args.steps):
+        for i in range(nkern):
+            per_kind[kinds[i]].append(events[s][i].elapsed_ms(events[s][i + 1]))
+    return w, gpu_out, total_ms / args.steps, per_kind, nkern
+
+
+def suite(libs, stream, steps):
+    """Config 2: forward / inverse / coset at 2^20, 2^22, 2^24 and the 4x extended coset of 2^22."""
+    from zkpoly_amd import hip
+    from zkpoly_amd.field import mont_limbs
+    from zkpoly_amd.ntt import gen_powers
+
+    out = {}
+    for k in (20, 22, 24):
+        w = Ntt(libs, stream, k, seed=3000 + k)
+        n = w.n
+        gb = hip.DeviceBuffer(32)
+        gb.upload(mont_limbs(7))
+        pw = hip.DeviceBuffer(n * 32)
+        gen_powers(libs, pw.addr, n, gb.addr, stream)
+        stream.sync()
+
+        def coset():
+            w.lib.distribute_powers(w.ptr, pw.addr, n, stream.raw())
+            w.ntt()
+
+        for name, fn in (("ntt", lambda: w.ntt()), ("intt", lambda: w.ntt(True)), ("coset_ntt", coset)):
+            ms = time_loop(stream, fn, steps, 2)
+            out[f"{name}_2^{k}"] = {"ms": round(ms, 4), "gelem_s": round(n / ms / 1e6, 3)}
+        del w, pw
+    # 4x extended coset of a 2^22 polynomial: zero-pad to 2^24, coset g = 7, NTT on 2^24
+    w = Ntt(libs, stream, 24, seed=4422)
+    n = w.n
+    w.host[1 << 22 :] = 0  # coefficients of a 2^22 polynomial, zero-padded to 2^24
+    w.reset()
+    gb = hip.DeviceBuffer(32)
+    gb.upload(mont_limbs(7))
+    pw = hip.DeviceBuffer(n * 32)
+    gen_powers(libs, pw.addr, n, gb.addr, stream)
+    stream.sync()
+
+    def ext():
+        w.lib.distribute_powers(w.ptr, pw.addr, n, stream.raw())
+        w.ntt()
+
+    ms = time_loop(stream, ext, steps, 2)
+    out["extended_coset_4x_2^22"] = {"ms": round(ms, 4), "gelem_s_out": round(n / ms / 1e6, 3)}
+    return out
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch from the committed PMC summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
+            "(multi-GPU sharding is not implemented)",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    from zkpoly_amd import hip
+    from zkpoly_amd.runtime import Libs
+
+    if hip.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    libs = Libs()
+    stream = hip.Stream(0)
+    k = args.log_n
+    n = 1 << k
+    log(f"[bench] headline: forward NTT 2^{k}, warmup {args.warmup}, steps {args.steps}")
+    w, gpu_out, ms_step, per_kind, nkern = headline(libs, stream, args)
+    dit = per_kind[1]
+    dit_avg_ms = sum(dit) / len(dit)
+    bytes_alg = 64 * n
+    achieved = bytes_alg / (dit_avg_ms * 1e-3) / 1e9
+    mads_total = MADS_PER_MODMUL * (n // 2) * k
+    int_rate = mads_total / (sum(dit) / args.steps * 1e-3)
+    line = {
+        "metric": "BN254 NTT 2^24 Gelem/s; MSM 2^20 ms; batched vs plain speedup, bit-exact",
+        "value": round(n / ms_step / 1e6, 4),
+        "unit": "Gelem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32x8 Montgomery (BN254 Fr)",
+        "data": "synthetic uniform Fr elements, PCG64 seed 2024, resident in HBM",
+        "config": {
+            "workload": f"in-place forward NTT of 2^{k} BN254 Fr elements (natural order), ssip_ntt C ABI",
+            "log_n": k,
+            "kernels_per_step": nkern,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "dit_pass",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles("dit_pass"),
+            "bytes_alg_per_launch": bytes_alg,
+            "avg_launch_ms": round(dit_avg_ms, 5),
+            "launches": len(dit),
+        },
+        "int_mul": {
+            "note": "the NTT is integer-multiply bound: 136 v_mad_u64_u32 per Montgomery product",
+            "achieved_mads_per_s": float(f"{int_rate:.4e}"),
+            "peak_mads_per_s": INT_MUL_PEAK,
+            "frac": round(int_rate / INT_MUL_PEAK, 4),
+        },
+        "bitrev_avg_ms": round(sum(per_kind[0]) / max(1, len(per_kind[0])), 5),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: oracle best_fft 2^{k} on {threads} threads")
+        t = time.perf_counter()
+        want = orc.ntt(w.host, k, threads)
+        cpu_s = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": round(n / cpu_s / 1e9, 5),
+            "unit": "Gelem/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"one forward NTT of the same 2^{k} input (best_fft restatement, 4x64-bit Montgomery)",
+            "seconds": round(cpu_s, 3),
+        }
+        line["bit_exact"] = bool(np.array_equal(gpu_out, want))
+    if not args.quick:
+        log("[bench] config-2 suite")
+        del w
+        line["suite"] = suite(libs, stream, max(5, args.steps // 2))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -217,6 +217,56 @@ static inline unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
   return (unsigned)g;
 }
 
+__global__ void __launch_bounds__(THREADS)
+    copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Validates, then launches bitrev + the planned passes on `stream`.  When
+// `events` is given, events[0] is recorded first and events[i + 1] after
+// kernel i (bounded by max_events).
+static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
+                             hipEvent_t* events, int max_events, int* kinds, int* num_kernels) {
+  PolyView v;
+  if (!make_view(x, &v)) return hipErrorInvalidValue;
+  if (log_len > 40 || v.len != (1ull << log_len)) return hipErrorInvalidValue;
+  if (log_len == 0) return hipSuccess;
+  if (v.ptr == nullptr || twiddle == nullptr) return hipErrorInvalidValue;
+  int ev = 0, nk = 0;
+  hipError_t err;
+  if (events && ev < max_events) {
+    err = hipEventRecord(events[ev++], stream);
+    if (err != hipSuccess) return err;
+  }
+  hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
+                     (uint32_t)log_len);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  if (kinds && nk < max_events) kinds[nk] = 0;
+  nk++;
+  if (events && ev < max_events) {
+    err = hipEventRecord(events[ev++], stream);
+    if (err != hipSuccess) return err;
+  }
+  PassPlan plan[48];
+  const int np = plan_passes(log_len, plan);
+  for (int p = 0; p < np; p++) {
+    hipLaunchKernelGGL(dit_pass, dim3((unsigned)plan[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
+                       (uint32_t)log_len, plan[p].s0, plan[p].d, plan[p].c_log);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    if (kinds && nk < max_events) kinds[nk] = 1;
+    nk++;
+    if (events && ev < max_events) {
+      err = hipEventRecord(events[ev++], stream);
+      if (err != hipSuccess) return err;
+    }
+  }
+  if (num_kernels) *num_kernels = nk;
+  return hipSuccess;
+}
+
 }  // namespace ntt
 }  // namespace zkp
 
@@ -233,24 +283,30 @@ hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned int log_len
                     const unsigned int max_threads_stage1_log, const unsigned int max_threads_stage2_log) {
   (void)max_threads_stage1_log;
   (void)max_threads_stage2_log;
-  PolyView v;
-  if (!make_view(x, &v)) return hipErrorInvalidValue;
-  if (log_len > 40 || v.len != (1ull << log_len)) return hipErrorInvalidValue;
-  if (log_len == 0) return hipSuccess;
-  if (v.ptr == nullptr || twiddle == nullptr) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
-                     (uint32_t)log_len);
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  PassPlan plan[48];
-  const int np = plan_passes(log_len, plan);
-  for (int p = 0; p < np; p++) {
-    hipLaunchKernelGGL(dit_pass, dim3((unsigned)plan[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
-                       (uint32_t)log_len, plan[p].s0, plan[p].d, plan[p].c_log);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-  }
-  return hipSuccess;
+  return zkp::ntt::launch_ntt(x, twiddle, log_len, stream, nullptr, 0, nullptr, nullptr);
+}
+
+// ssip_ntt that also records events[0] before the first kernel and
+// events[i + 1] after kernel i (i < *num_kernels), for per-kernel timing of the
+// same launches.  kinds[i] = 0 for the bit-reversal, 1 for a butterfly pass.
+// No counterpart in the reference ABI (measurement hook).
+hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
+                               hipEvent_t* events, int max_events, int* kinds, int* num_kernels) {
+  int nk = 0;
+  hipError_t err = zkp::ntt::launch_ntt(x, twiddle, log_len, stream, events, max_events, kinds, &nk);
+  if (num_kernels) *num_kernels = nk;
+  return err;
+}
+
+// dst[i] = src[i] for n 16-byte words: a plain streaming copy used to calibrate
+// the HBM traffic counters (FETCH_SIZE / WRITE_SIZE) against known bytes.
+hipError_t zkp_copy_u4(void* dst, const void* src, unsigned long long n16, hipStream_t stream) {
+  if (n16 == 0) return hipSuccess;
+  if (dst == nullptr || src == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(zkp::ntt::copy_u4_kernel, dim3(zkp::ntt::grid_for(n16, zkp::ntt::THREADS, 65536)),
+                     dim3(zkp::ntt::THREADS), 0, stream, reinterpret_cast<uint4*>(dst),
+                     reinterpret_cast<const uint4*>(src), (uint64_t)n16);
+  return hipGetLastError();
 }
 
 // twiddle[i] = unit^i for i < len, written to HOST memory; synchronous.
@@ -277,9 +333,9 @@ hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len, const unsign
   }
   if (err == hipSuccess) err = hipMemcpyAsync(twiddle, d_out, (size_t)len * 32, hipMemcpyDeviceToHost, s);
   if (err == hipSuccess) err = hipStreamSynchronize(s);
-  if (d_out) hipFree(d_out);
-  if (d_unit) hipFree(d_unit);
-  hipStreamDestroy(s);
+  if (d_out) (void)hipFree(d_out);
+  if (d_unit) (void)hipFree(d_unit);
+  (void)hipStreamDestroy(s);
   return err;
 }
 
