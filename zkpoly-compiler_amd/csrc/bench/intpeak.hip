@@ -117,6 +117,16 @@ __global__ void k_frmul(u64* out, u32 y, int iters) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// One dependent chain in one wave: latency of a Montgomery product.
+__global__ void k_frmul_latency(u64* out, u32 y, int iters) {
+  Fr acc = Fr::one();
+  Fr m = Fr::one();
+  m.v[0] ^= y;
+  acc.v[1] ^= threadIdx.x;
+  for (int it = 0; it < iters; it++) acc = acc * m;
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc.v[0] ^ acc.v[7];
+}
+
 typedef void (*kern_t)(u64*, u32, int);
 
 static void run(const char* name, kern_t k, int iters, double ops_per_iter, int blocks,
@@ -162,6 +172,22 @@ int main(int argc, char** argv) {
   run("add_u64", k_add64, 4096, CHAINS, blocks, threads, dout);
   run("v_fma_f64", k_fma64, 4096, CHAINS, blocks, threads, dout);
   run("fr_mont_mul", k_frmul, 256, 4, blocks, threads, dout);
+  {
+    // latency: a single wave, one chain
+    hipEvent_t a, b;
+    CHECK(hipEventCreate(&a));
+    CHECK(hipEventCreate(&b));
+    const int iters = 20000;
+    hipLaunchKernelGGL(k_frmul_latency, dim3(1), dim3(64), 0, 0, dout, 0x9e3779b9u, iters);
+    CHECK(hipDeviceSynchronize());
+    CHECK(hipEventRecord(a, 0));
+    hipLaunchKernelGGL(k_frmul_latency, dim3(1), dim3(64), 0, 0, dout, 0x9e3779b9u, iters);
+    CHECK(hipEventRecord(b, 0));
+    CHECK(hipEventSynchronize(b));
+    float ms;
+    CHECK(hipEventElapsedTime(&ms, a, b));
+    printf("{\"kernel\": \"fr_mont_mul_latency_one_wave\", \"ns_per_mul\": %.2f}\n", ms * 1e6 / iters);
+  }
   CHECK(hipFree(dout));
   return 0;
 }

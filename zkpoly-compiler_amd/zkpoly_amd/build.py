@@ -30,9 +30,20 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-# MSM window sizes shipped prebuilt (libmsm_bn254_<w>_<w>_false.so); other
-# configurations are built on demand by msm.ensure_msm_lib().
-PREBUILT_MSM_WINDOWS = (12, 13, 14, 15, 16, 17, 18)
+# MsmConfig (window, target_windows) pairs shipped prebuilt as
+# libmsm_bn254_<w>_<t>_false.so: windows 12..18 without precomputation
+# (target = ceil(254 / w)) and with one precomputed table (target = half).
+# Other configurations are built on demand by msm.ensure_msm_lib().
+def _prebuilt_msm_configs():
+    out = []
+    for w in range(12, 19):
+        windows = -(-254 // w)
+        out.append((w, windows))
+        out.append((w, -(-windows // 2)))
+    return tuple(sorted(set(out)))
+
+
+PREBUILT_MSM_CONFIGS = _prebuilt_msm_configs()
 
 
 def hipcc() -> str:
@@ -62,6 +73,45 @@ def build_ntt() -> str:
     return build_shared([os.path.join(CSRC, "ntt", "ntt.hip")], "libntt.so")
 
 
+def build_msm_core() -> str:
+    return build_shared([os.path.join(CSRC, "msm", "msm.hip")], "libzkp_msm.so")
+
+
+def msm_lib_name(window: int, target: int, debug: bool = False) -> str:
+    """Library name the reference derives from an MsmConfig (core/src/msm.rs:100-111)."""
+    return f"libmsm_bn254_{window}_{target}_{'true' if debug else 'false'}.so"
+
+
+def build_msm_abi(window: int, target: int, debug: bool = False) -> str:
+    """Host-only wrapper binding msm / msm_precompute to one configuration."""
+    os.makedirs(LIB, exist_ok=True)
+    name = msm_lib_name(window, target, debug)
+    out = os.path.join(LIB, name)
+    tmp = out + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [
+        shutil.which("g++") or "g++",
+        "-O2",
+        "-fPIC",
+        "-shared",
+        "-std=c++17",
+        "-D__HIP_PLATFORM_AMD__",
+        f"-I{rocm}/include",
+        f"-DZKP_MSM_WINDOW={window}",
+        f"-DZKP_MSM_TARGET={target}",
+        f"-DZKP_MSM_DEBUG={1 if debug else 0}",
+        os.path.join(CSRC, "msm", "msm_abi.cpp"),
+        f"-L{LIB}",
+        "-lzkp_msm",
+        "-Wl,-rpath,$ORIGIN",
+        "-o",
+        tmp,
+    ]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
 def build_intpeak() -> str:
     os.makedirs(os.path.join(PACKAGE_ROOT, "build"), exist_ok=True)
     out = os.path.join(PACKAGE_ROOT, "build", "intpeak")
@@ -70,14 +120,18 @@ def build_intpeak() -> str:
 
 
 def targets() -> list:
-    return [build_ntt, build_intpeak]
+    return [build_ntt, build_msm_core, build_intpeak]
 
 
 def build_all(jobs: int | None = None) -> list[str]:
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(t) for t in targets()]
-        return [f.result() for f in futs]
+        out = [f.result() for f in futs]
+        # per-configuration MSM wrappers link against libzkp_msm.so
+        futs = [ex.submit(build_msm_abi, w, t, False) for (w, t) in PREBUILT_MSM_CONFIGS]
+        out += [f.result() for f in futs]
+    return out
 
 
 if __name__ == "__main__":
