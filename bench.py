@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline NTT only (no suite / MSM / batched)")
+    ap.add_argument("--msm-only", action="store_true", help="MSM 2^20 leg only (for profiling)")
+    ap.add_argument("--msm-log-n", type=int, default=20)
     return ap.parse_args()
 
 
@@ -199,6 +201,73 @@ def suite(libs, stream, steps):
     return out
 
 
+def msm_leg(libs, stream, log_n, steps, with_cpu):
+    """Config 3: MSM of 2^log_n points P_i = k_i G, window sweep 12..18, uniform and 10 % / 10 % scalars."""
+    from zkpoly_amd import hip
+    from zkpoly_amd.field import mont_limbs, random_elements
+    from zkpoly_amd.msm import msm_device, mul_generator, workspace_size
+
+    n = 1 << log_n
+    ks = random_elements(n, 3000)
+    ss = random_elements(n, 3001)
+    rng = np.random.Generator(np.random.PCG64(3002))
+    perm = rng.permutation(n)
+    ss01 = ss.copy()
+    ss01[perm[: n // 10]] = 0
+    ss01[perm[n // 10 : n // 5]] = mont_limbs(1)
+    kb = hip.DeviceBuffer(n * 32)
+    kb.upload(ks)
+    pb = hip.DeviceBuffer(n * 64)
+    mul_generator(libs, kb.addr, n, pb.addr, stream)
+    sb = hip.DeviceBuffer(n * 32)
+    sb.upload(ss)
+    sb01 = hip.DeviceBuffer(n * 32)
+    sb01.upload(ss01)
+    stream.sync()
+    out = {"log_n": log_n, "windows": {}}
+    results = {}
+    for window in range(12, 19):
+        target = -(-254 // window)
+        ws = hip.DeviceBuffer(workspace_size(libs, n, window, target))
+        for scal, tag in ((sb, "uniform"), (sb01, "zeros_ones")):
+            if tag == "zeros_ones" and window != 16:
+                continue
+            r = msm_device(libs, pb.addr, scal.addr, n, window, target, ws, stream)
+            times = []
+            for _ in range(steps):
+                t = time.perf_counter()
+                r = msm_device(libs, pb.addr, scal.addr, n, window, target, ws, stream)
+                times.append((time.perf_counter() - t) * 1e3)
+            times.sort()
+            key = f"w{window}" if tag == "uniform" else f"w{window}_zeros_ones"
+            out["windows"][key] = {"ms_median": round(times[len(times) // 2], 4), "ms_min": round(times[0], 4)}
+            results[(window, tag)] = r
+        del ws
+    best = min((v["ms_median"], k) for k, v in out["windows"].items() if "zeros" not in k)
+    out["best_window"] = best[1]
+    out["ms"] = best[0]
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        from zkpoly_amd.field import from_mont
+
+        want = {}
+        for tag, arr in (("uniform", ss), ("zeros_ones", ss01)):
+            total = from_mont(orc.to_int(orc.inner_product(ks, arr)))
+            want[tag] = list(orc.g1_mul_canonical(orc.generator(), total))
+        out["bit_exact"] = all(list(r) == want[tag] for (w, tag), r in results.items())
+        threads = min(16, os.cpu_count() or 1)
+        bases = pb.download(np.uint64, n * 8).reshape(n, 8)
+        log(f"[bench] cpu baseline: oracle best_multiexp 2^{log_n} on {threads} threads")
+        t = time.perf_counter()
+        cpu = orc.best_multiexp(ss, bases, threads)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"ms": round(cpu_s * 1e3, 2), "cores": threads, "kind": "port",
+                               "sample": f"best_multiexp restatement on the same 2^{log_n} uniform instance"}
+        out["bit_exact"] = out["bit_exact"] and list(cpu) == want["uniform"]
+    return out
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -226,6 +295,10 @@ def main():
         sys.exit(1)
     libs = Libs()
     stream = hip.Stream(0)
+    if args.msm_only:
+        log(f"[bench] MSM 2^{args.msm_log_n} only")
+        print(json.dumps({"msm": msm_leg(libs, stream, args.msm_log_n, args.steps, not args.no_cpu)}), flush=True)
+        return
     k = args.log_n
     n = 1 << k
     log(f"[bench] headline: forward NTT 2^{k}, warmup {args.warmup}, steps {args.steps}")
@@ -296,6 +369,8 @@ def main():
         log("[bench] config-2 suite")
         del w
         line["suite"] = suite(libs, stream, max(5, args.steps // 2))
+        log("[bench] MSM 2^20 window sweep")
+        line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

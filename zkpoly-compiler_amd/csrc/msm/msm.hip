@@ -103,19 +103,52 @@ __device__ __forceinline__ int next_digit(const Fr& c, uint32_t w, uint32_t s, i
   return d;
 }
 
+// Wave-aggregated atomic increment of counter[bin] for the active lanes;
+// returns each lane's previous value plus its rank among lanes with the same
+// bin.  Up to 4 rounds serve the bin of the lowest active lane with a single
+// atomic for all lanes sharing it; what is left (all-distinct bins) falls back
+// to one atomic per lane.  This removes the serialisation on the few bins of a
+// narrow top window (e.g. bits 252..253 for s = 12, 14, 18) and on a bucket that
+// receives a fixed share of all scalars.  Must be called by the whole wave.
+__device__ __forceinline__ uint32_t aggregated_increment(uint32_t* counter, uint32_t bin, bool active) {
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t result = 0;
+  for (int round = 0; round < 4; round++) {
+    const uint64_t act = __ballot(active);
+    if (act == 0) return result;
+    const int leader = __ffsll((long long)act) - 1;
+    const uint32_t lbin = __shfl(bin, leader);
+    const uint64_t same = __ballot(active && bin == lbin);
+    if (round > 0 && __popcll(same) < 2) break;
+    uint32_t start = 0;
+    if ((int)lane == leader) start = atomicAdd(&counter[lbin], (uint32_t)__popcll(same));
+    start = __shfl(start, leader);
+    if (active && bin == lbin) {
+      result = start + (uint32_t)__popcll(same & below);
+      active = false;
+    }
+  }
+  if (active) result = atomicAdd(&counter[bin], 1u);
+  return result;
+}
+
+// All lanes of a wave run the same number of iterations (inactive tails keep
+// participating in the wave-wide ballots).
 __global__ void __launch_bounds__(THREADS)
     k_histogram(const uint32_t* __restrict__ scalars, uint64_t n, Config cfg, uint32_t* __restrict__ hist) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const Fr c = Fr::load(scalars + i * 8).to_canonical();
+  const uint64_t rounds = (n + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; r++) {
+    const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const Fr c = valid ? Fr::load(scalars + i * 8).to_canonical() : Fr::zero();
     int carry = 0;
     for (uint32_t w = 0; w < cfg.actual_windows; w++) {
       const int d = next_digit(c, w, cfg.s, carry);
-      if (d != 0) {
-        const uint32_t pw = w % cfg.n_windows;
-        const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
-        atomicAdd(&hist[pw * cfg.buckets + b], 1u);
-      }
+      const uint32_t pw = w % cfg.n_windows;
+      const uint32_t bin = pw * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+      aggregated_increment(hist, bin, valid && d != 0);
     }
   }
 }
@@ -124,16 +157,20 @@ __global__ void __launch_bounds__(THREADS)
     k_scatter(const uint32_t* __restrict__ scalars, uint64_t n, Config cfg, uint32_t* __restrict__ cursor,
               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const Fr c = Fr::load(scalars + i * 8).to_canonical();
+  const uint64_t rounds = (n + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; r++) {
+    const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const Fr c = valid ? Fr::load(scalars + i * 8).to_canonical() : Fr::zero();
     int carry = 0;
     for (uint32_t w = 0; w < cfg.actual_windows; w++) {
       const int d = next_digit(c, w, cfg.s, carry);
-      if (d != 0) {
-        const uint32_t pw = w % cfg.n_windows;
-        const uint32_t table = w / cfg.n_windows;
-        const uint32_t key = pw * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
-        const uint32_t pos = atomicAdd(&cursor[key], 1u);
+      const uint32_t pw = w % cfg.n_windows;
+      const uint32_t table = w / cfg.n_windows;
+      const uint32_t key = pw * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+      const bool active = valid && d != 0;
+      const uint32_t pos = aggregated_increment(cursor, key, active);
+      if (active) {
         keys[pos] = key;
         vals[pos] = (uint32_t)(table * n + i) | (d < 0 ? 0x80000000u : 0u);
       }
