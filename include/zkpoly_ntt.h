@@ -1,0 +1,52 @@
+/* libntt.so -- BN254 Fr NTT entry points (the reference's core/src/ntt/src/ntt.h:6-17).
+ *
+ * All data pointers are device pointers unless stated; asynchronous entry
+ * points run on `stream` and allocate nothing.
+ */
+#ifndef ZKPOLY_NTT_H
+#define ZKPOLY_NTT_H
+
+#include "zkpoly_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* library: libntt.so */
+
+/* In-place natural-order forward NTT of x (x.len = 2^log_len) with the twiddle
+ * table [1, w, ..., w^(len/2 - 1)].  Replaces ssip_ntt, core/src/ntt/src/ntt.h:6
+ * (bound by core/src/ntt.rs:24-37,165-200).  The two max_threads hints are
+ * accepted and ignored. */
+hipError_t ssip_ntt(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
+                    const unsigned int max_threads_stage1_log, const unsigned int max_threads_stage2_log);
+
+/* twiddle[i] = unit^i for i < len, into HOST memory; synchronous.  Replaces
+ * ssip_precompute, core/src/ntt/src/ntt.h:9 (bound by core/src/ntt.rs:39-50,204-238). */
+hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len, const unsigned int* unit);
+
+/* x[i] *= powers[i mod power_num].  Replaces distribute_powers,
+ * core/src/ntt/src/ntt.h:17 (bound by core/src/ntt.rs:69-80,112-150). */
+hipError_t distribute_powers(PolyPtr poly, const unsigned int* powers, unsigned long long power_num,
+                             hipStream_t stream);
+
+/* ---- extensions (no counterpart in the reference ABI) ---- */
+
+/* out[i] = base^i for i < len (device), asynchronous: coset power tables and
+ * twiddle tables built directly in HBM. */
+hipError_t zkp_gen_powers(unsigned int* out, unsigned long long len, const unsigned int* base, hipStream_t stream);
+
+/* ssip_ntt that also records events[0] before the first kernel and
+ * events[i + 1] after kernel i; kinds[i] = 0 (bit reversal) or 1 (butterfly
+ * pass); *num_kernels = kernels launched.  Measurement hook. */
+hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
+                               hipEvent_t* events, int max_events, int* kinds, int* num_kernels);
+
+/* dst[i] = src[i] for n16 16-byte words: plain copy used to calibrate HBM counters. */
+hipError_t zkp_copy_u4(void* dst, const void* src, unsigned long long n16, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKPOLY_NTT_H */

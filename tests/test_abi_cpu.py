@@ -1,0 +1,143 @@
+"""CPU checks of the drop-in boundary (no GPU needed, no kernels launched).
+
+* every function include/*.h declares is exported by the library its
+  "library:" section names;
+* PolyPtr has the reference's 40-byte #[repr(C)] layout (core/src/poly_ptr.rs:4-11);
+* host-side validation returns hipErrorInvalidValue before touching a device
+  (the reference's Rust asserts, core/src/ntt.rs:173-179, become error codes);
+* ScalarArray rotation / slice index maps follow runtime/src/scalar.rs:351-444;
+* MsmConfig and library naming follow common/src/msm_config.rs:14-24 and
+  core/src/msm.rs:100-111; the MSM sizing call works without a device.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from zkpoly_amd import build
+from zkpoly_amd.msm import MSM, MsmConfig, get_msm_lib_name
+from zkpoly_amd.runtime import Libs, PolyPtr, ScalarArray
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+HIP_INVALID_VALUE = 1
+
+DECL = re.compile(r"^\s*(?:hipError_t|unsigned long long|int|void)\s+(\w+)\s*\(", re.M)
+
+
+def header_sections():
+    """(header, library, [function names]) for every library: section."""
+    out = []
+    for h in sorted(os.listdir(INCLUDE)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        parts = re.split(r"/\*\s*library:\s*(\S+)\s*\*/", text)
+        for i in range(1, len(parts), 2):
+            lib = parts[i].replace("<w>_<t>_<dbg>", "16_16_false")
+            names = DECL.findall(parts[i + 1])
+            out.append((h, lib, names))
+    return out
+
+
+def exported(path):
+    r = subprocess.run(["nm", "-D", "--defined-only", path], stdout=subprocess.PIPE, text=True, check=True)
+    return {line.split()[-1] for line in r.stdout.splitlines() if line.strip()}
+
+
+def test_headers_declare_something():
+    secs = header_sections()
+    assert {s[1] for s in secs} >= {"libntt.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so"}
+    assert all(s[2] for s in secs)
+
+
+@pytest.mark.parametrize("header,lib,names", header_sections())
+def test_library_exports_every_declared_symbol(header, lib, names):
+    path = os.path.join(build.LIB, lib)
+    assert os.path.exists(path), f"{lib} not built"
+    ctypes.CDLL(path)  # loads without a GPU
+    missing = sorted(set(names) - exported(path))
+    assert not missing, f"{lib} lacks {missing} declared in {header}"
+
+
+def test_polyptr_layout():
+    assert ctypes.sizeof(PolyPtr) == 40
+    assert [PolyPtr.__dict__[f].offset for f in ("ptr", "len", "rotate", "offset", "whole_len")] == [0, 8, 16, 24, 32]
+
+
+def _ntt():
+    from zkpoly_amd.ntt import load_lib
+
+    return load_lib(Libs())
+
+
+def test_ssip_ntt_rejects_bad_shapes_without_launch():
+    lib = _ntt()
+    # len not 2^log_len
+    assert lib.ssip_ntt(PolyPtr(0x1000, 12, 0, 0, 12), 0x2000, 3, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.ssip_ntt(PolyPtr(0x1000, 16, 0, 0, 16), 0x2000, 3, None, 8, 8) == HIP_INVALID_VALUE
+    # null buffers
+    assert lib.ssip_ntt(PolyPtr(0, 16, 0, 0, 16), 0x2000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.ssip_ntt(PolyPtr(0x1000, 16, 0, 0, 16), 0, 4, None, 8, 8) == HIP_INVALID_VALUE
+    # rotation on a slice, slice longer than its array, offset out of range
+    assert lib.ssip_ntt(PolyPtr(0x1000, 16, 3, 5, 32), 0x2000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.ssip_ntt(PolyPtr(0x1000, 16, 0, 0, 8), 0x2000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.ssip_ntt(PolyPtr(0x1000, 16, 0, 32, 32), 0x2000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    # a length-1 transform is the identity and launches nothing
+    assert lib.ssip_ntt(PolyPtr(0x1000, 1, 0, 0, 1), 0x2000, 0, None, 8, 8) == 0
+
+
+def test_distribute_powers_rejects_bad_args():
+    lib = _ntt()
+    assert lib.distribute_powers(PolyPtr(0x1000, 16, 0, 0, 16), 0x2000, 0, None) == HIP_INVALID_VALUE
+    assert lib.distribute_powers(PolyPtr(0x1000, 16, 0, 0, 16), 0, 3, None) == HIP_INVALID_VALUE
+    assert lib.distribute_powers(PolyPtr(0x1000, 0, 0, 0, 0), 0x2000, 3, None) == 0
+
+
+def test_scalar_array_index_maps():
+    # rotate(3) then slice(n, 2n) of a 2n array (core/tests/ntt.rs:82-87)
+    n = 16
+    a = ScalarArray(2 * n, 0)
+    a.rotate(3)
+    s = a.slice(n, 2 * n)
+    assert s.slice_info.offset == (2 * n + n - 3) % (2 * n)
+    assert [s.physical_index(i) for i in range(n)] == [(n - 3 + i) % (2 * n) for i in range(n)]
+    p = PolyPtr.from_array(s)
+    assert (p.len, p.rotate, p.offset, p.whole_len) == (n, 0, n - 3, 2 * n)
+    # rotate(-2) of an unsliced array: logical i at (i + 2) mod n, PolyPtr offset = len - rotate
+    b = ScalarArray(n, 0)
+    b.rotate(-2)
+    assert b.get_rotation() == n - 2
+    assert [b.physical_index(i) for i in range(n)] == [(i + 2) % n for i in range(n)]
+    q = PolyPtr.from_array(b)
+    assert (q.len - q.rotate) % n == 2
+
+
+def test_msm_config_and_names():
+    assert MsmConfig().get_precompute() == 1
+    assert MsmConfig(window_size=16, target_window=8).get_precompute() == 2
+    assert MsmConfig(window_size=13, target_window=5).get_precompute() == 4
+    assert get_msm_lib_name(MsmConfig(window_size=16, target_window=16)) == "libmsm_bn254_16_16_false.so"
+    assert get_msm_lib_name(MsmConfig(window_size=12, target_window=4, debug=True)) == "libmsm_bn254_12_4_true.so"
+
+
+def test_msm_sizing_call_without_device():
+    op = MSM(Libs(), MsmConfig(window_size=16, target_window=16))
+    small, big = op.get_buffer_size(1 << 10)[0], op.get_buffer_size(1 << 20)[0]
+    assert 0 < small < big
+    # two parts need less device memory than one
+    op2 = MSM(Libs(), MsmConfig(window_size=16, target_window=16, parts=2))
+    assert op2.get_buffer_size(1 << 20)[0] < big
+
+
+def test_msm_config_query():
+    from zkpoly_amd.msm import core_lib
+
+    lib = core_lib(Libs())
+    out = (ctypes.c_uint * 5)()
+    assert lib.zkp_msm_config(16, 16, out) == 1
+    assert list(out) == [16, 16, 16, 1, 1 << 15]
+    assert lib.zkp_msm_config(1, 16, out) == 0
+    assert lib.zkp_msm_workspace_size(1 << 20, 16, 16) > 0
