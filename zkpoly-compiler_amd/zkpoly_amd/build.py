@@ -112,6 +112,25 @@ def build_msm_abi(window: int, target: int, debug: bool = False) -> str:
     return out
 
 
+def build_fused_codegen() -> str:
+    """The C++ ArithGraph -> HIP generator (host-only library)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libzkp_fused_codegen.so")
+    tmp = out + ".tmp"
+    _run([shutil.which("g++") or "g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall",
+          os.path.join(CSRC, "fused", "codegen.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fused_demo() -> str:
+    """Prebuilt fused-kernel library (the workload's a*b + gamma*c - z and test graphs)."""
+    from . import fused
+    from .runtime import Libs
+
+    return fused.build_library(Libs(), fused.demo_kernels(), fused.DEMO_DIR)
+
+
 def build_intpeak() -> str:
     os.makedirs(os.path.join(PACKAGE_ROOT, "build"), exist_ok=True)
     out = os.path.join(PACKAGE_ROOT, "build", "intpeak")
@@ -120,7 +139,7 @@ def build_intpeak() -> str:
 
 
 def targets() -> list:
-    return [build_ntt, build_msm_core, build_intpeak]
+    return [build_ntt, build_msm_core, build_fused_codegen, build_intpeak]
 
 
 def build_all(jobs: int | None = None) -> list[str]:
@@ -128,8 +147,10 @@ def build_all(jobs: int | None = None) -> list[str]:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(t) for t in targets()]
         out = [f.result() for f in futs]
-        # per-configuration MSM wrappers link against libzkp_msm.so
+        # per-configuration MSM wrappers link against libzkp_msm.so; the fused
+        # demo library needs the generator
         futs = [ex.submit(build_msm_abi, w, t, False) for (w, t) in PREBUILT_MSM_CONFIGS]
+        futs.append(ex.submit(build_fused_demo))
         out += [f.result() for f in futs]
     return out
 

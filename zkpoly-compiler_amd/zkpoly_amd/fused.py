@@ -1,0 +1,280 @@
+"""Fused elementwise kernels: ArithGraph JSON -> HIP library -> operator.
+
+Mirrors (reference paths):
+  * FusedOp::gen / gen_fused_kernels: ArithGraph -> kernel source + library
+    (core/src/fused_kernels/kernel_gen.rs:47-191,
+    compiler/src/transit/type3/lowering/kernel_gen.rs:96-158); here the C++
+    generator in libzkp_fused_codegen.so emits HIP and runs hipcc.
+  * FusedKernelMeta {name, num_vars, num_mut_vars, lib_path}
+    (runtime/src/functions.rs:37-43).
+  * FusedKernel::get_fn (core/src/fused_kernels.rs:97-196): vars =
+    [stream, inputs...], mut_vars = [arg_buffer, local_buffer, outputs...];
+    the PolyPtr arrays are copied into arg_buffer and the kernel is called
+    with is_first = true.
+GraphBuilder writes the serde JSON of common::arith::ArithGraph
+(common/src/arith.rs:172-331) so graphs can be made without the compiler.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import hip
+from .runtime import (
+    PACKAGE_ROOT,
+    FuncMeta,
+    Function,
+    GpuBuffer,
+    KernelType,
+    Libs,
+    PolyPtr,
+    RegisteredFunction,
+    Scalar,
+    ScalarArray,
+    unwrap_stream,
+)
+
+CODEGEN_LIB = "libzkp_fused_codegen.so"
+LIB_NAME = "libfused_kernels.so"
+POLYPTR_BYTES = ctypes.sizeof(PolyPtr)
+
+
+class GraphBuilder:
+    """Builds the serde JSON form of ArithGraph<VariableId, ExprId>."""
+
+    def __init__(self, poly_repr: str = "Lagrange", poly_degree: Optional[int] = None):
+        assert poly_repr in ("Lagrange", "Coef")
+        self.vertices: list = []
+        self.inputs: list[int] = []
+        self.outputs: list[int] = []
+        self.poly_repr = poly_repr
+        self.poly_degree = poly_degree
+        self._outer = 0
+
+    def _add(self, op) -> int:
+        self.vertices.append({"op": op})
+        return len(self.vertices) - 1
+
+    def input(self, scalar: bool = False, mutable: bool = False) -> int:
+        vid = self._add({"Input": {"outer_id": self._outer, "typ": "Scalar" if scalar else "ScalarArray",
+                                   "mutability": "Mut" if mutable else "Const"}})
+        self._outer += 1
+        self.inputs.append(vid)
+        return vid
+
+    def bin(self, group: str, op: str, lhs: int, rhs: int) -> int:
+        assert group in ("Pp", "Ss", "Sp")
+        return self._add({"Arith": {"Bin": [{group: op}, lhs, rhs]}})
+
+    def neg(self, x: int, scalar: bool = False) -> int:
+        return self._add({"Arith": {"Unr": [{"S" if scalar else "P": "Neg"}, x]}})
+
+    def pow(self, x: int, e: int) -> int:
+        return self._add({"Arith": {"Unr": [{"P": {"Pow": e}}, x]}})
+
+    def output(self, store: int, scalar: bool = False, in_node: Optional[int] = None) -> int:
+        vid = self._add({"Output": {"outer_id": self._outer, "typ": "Scalar" if scalar else "ScalarArray",
+                                    "store_node": store, "in_node": in_node}})
+        self._outer += 1
+        self.outputs.append(vid)
+        return vid
+
+    def to_json(self) -> str:
+        return json.dumps({"outputs": self.outputs, "inputs": self.inputs, "g": [self.vertices, None],
+                           "poly_repr": self.poly_repr, "poly_degree": self.poly_degree})
+
+
+def graph_ab_gamma_c_minus_z() -> str:
+    """out = a*b + gamma*c - z over Lagrange polynomials a, b, c and scalars gamma, z.
+
+    vars order: [a, b, c, gamma, z]; mut_vars: [out].
+    """
+    g = GraphBuilder("Lagrange")
+    a, b, c = g.input(), g.input(), g.input()
+    gamma, z = g.input(scalar=True), g.input(scalar=True)
+    ab = g.bin("Pp", "Mul", a, b)
+    gc = g.bin("Sp", "Mul", gamma, c)
+    s = g.bin("Pp", "Add", ab, gc)
+    r = g.bin("Sp", "SubBy", z, s)  # s - z
+    g.output(r)
+    return g.to_json()
+
+
+def graph_coef_rules() -> str:
+    """Coefficient-form scalar-polynomial ops (kernel_gen.rs:503-534).
+
+    vars [p, s]; mut_vars [s + p, s - p, p - s, s * p].
+    """
+    g = GraphBuilder("Coef")
+    p, s = g.input(), g.input(scalar=True)
+    for op in ("Add", "Sub", "SubBy", "Mul"):
+        g.output(g.bin("Sp", op, s, p))
+    return g.to_json()
+
+
+def graph_inplace_mixed() -> str:
+    """In-place output, scalar ops and a scalar output.
+
+    vars [y, s] (x is read through its in-place slot); mut_vars [x', d, t, q] with
+    x' = (x + y)^3 * s (overwrites x), d = -(x - y), t = s*s + s (scalar), q = y * t.
+    """
+    g = GraphBuilder("Lagrange")
+    x = g.input(mutable=True)
+    y = g.input()
+    s = g.input(scalar=True)
+    xy = g.bin("Pp", "Add", x, y)
+    cube = g.pow(xy, 3)
+    new_x = g.bin("Sp", "Mul", s, cube)
+    g.output(new_x, in_node=x)
+    g.output(g.neg(g.bin("Pp", "Sub", x, y)))
+    t = g.bin("Ss", "Add", g.bin("Ss", "Mul", s, s), s)
+    g.output(t, scalar=True)
+    g.output(g.bin("Sp", "Mul", t, y))
+    return g.to_json()
+
+
+def demo_kernels() -> list[tuple[str, str]]:
+    return [
+        (graph_ab_gamma_c_minus_z(), "fused_arith_abgcz"),
+        (graph_coef_rules(), "fused_arith_coef_rules"),
+        (graph_inplace_mixed(), "fused_arith_inplace_mixed"),
+    ]
+
+
+DEMO_DIR = os.path.join(PACKAGE_ROOT, "lib", "fused_demo")
+
+
+def _codegen_lib(libs: Libs) -> ctypes.CDLL:
+    lib = libs.load_relative(CODEGEN_LIB)
+    if not getattr(lib, "_zkp_typed", False):
+        P = ctypes.c_void_p
+        lib.zkp_fused_codegen.argtypes = [ctypes.c_char_p, ctypes.c_char_p, P, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t), ctypes.c_char_p, ctypes.c_size_t]
+        lib.zkp_fused_codegen.restype = ctypes.c_int
+        lib.zkp_fused_build.argtypes = [P, P, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                        P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        lib.zkp_fused_build.restype = ctypes.c_int
+        lib._zkp_typed = True
+    return lib
+
+
+class CodegenError(ValueError):
+    pass
+
+
+def codegen(libs: Libs, graph_json: str, name: str) -> str:
+    lib = _codegen_lib(libs)
+    err = ctypes.create_string_buffer(1024)
+    n = ctypes.c_size_t(0)
+    if lib.zkp_fused_codegen(graph_json.encode(), name.encode(), None, 0, ctypes.byref(n), err, 1024) != 0:
+        raise CodegenError(err.value.decode())
+    buf = ctypes.create_string_buffer(n.value + 1)
+    lib.zkp_fused_codegen(graph_json.encode(), name.encode(), buf, n.value + 1, None, err, 1024)
+    return buf.value.decode()
+
+
+def build_library(libs: Libs, kernels: list[tuple[str, str]], out_dir: str) -> str:
+    """Generates and compiles [(graph_json, name), ...] into out_dir/libfused_kernels.so."""
+    from . import build
+
+    lib = _codegen_lib(libs)
+    jsons = (ctypes.c_char_p * len(kernels))(*[k[0].encode() for k in kernels])
+    names = (ctypes.c_char_p * len(kernels))(*[k[1].encode() for k in kernels])
+    err = ctypes.create_string_buffer(4096)
+    os.makedirs(out_dir, exist_ok=True)
+    rc = lib.zkp_fused_build(jsons, names, len(kernels), out_dir.encode(), build.hipcc().encode(),
+                             os.path.join(PACKAGE_ROOT, "csrc").encode(), None, 0, err, 4096)
+    if rc != 0:
+        raise CodegenError(err.value.decode())
+    return os.path.join(out_dir, LIB_NAME)
+
+
+@dataclass
+class FusedKernelMeta:
+    name: str
+    num_vars: int
+    num_mut_vars: int
+    lib_path: str
+
+
+def arg_buffer_bytes(meta: FusedKernelMeta) -> int:
+    """Size of the argument buffer the compiler allocates (temporary_space.rs:49-56)."""
+    return (meta.num_vars + meta.num_mut_vars) * POLYPTR_BYTES
+
+
+def _load(meta: FusedKernelMeta) -> ctypes.CDLL:
+    path = os.path.join(meta.lib_path, LIB_NAME)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built")
+    lib = ctypes.CDLL(path)
+    f = getattr(lib, meta.name)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_bool, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    fb = getattr(lib, meta.name + "_batched")
+    fb.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_ulonglong,
+                   ctypes.c_bool, ctypes.c_void_p]
+    fb.restype = ctypes.c_int
+    ar = getattr(lib, meta.name + "_arity")
+    ar.argtypes = [ctypes.c_void_p]
+    out = (ctypes.c_uint * 2)()
+    ar(out)
+    if (out[0], out[1]) != (meta.num_vars, meta.num_mut_vars):
+        raise ValueError(f"{meta.name}: library arity {tuple(out)} != meta ({meta.num_vars}, {meta.num_mut_vars})")
+    return lib
+
+
+def _ptr_of(v) -> PolyPtr:
+    if isinstance(v, ScalarArray):
+        return PolyPtr.from_array(v)
+    if isinstance(v, Scalar):
+        return PolyPtr(v.value, 1, 0, 0, 1)
+    raise AssertionError("Only scalars and scalar arrays are supported")
+
+
+class FusedKernel(RegisteredFunction):
+    def __init__(self, libs: Libs, meta: FusedKernelMeta):
+        self.meta = meta
+        self.lib = _load(meta)
+        self.c_func = getattr(self.lib, meta.name)
+        self.c_batched = getattr(self.lib, meta.name + "_batched")
+
+    def get_fn(self) -> Function:
+        meta, c_func = self.meta, self.c_func
+
+        def f(mut_var, var, gpu_mapping=None):
+            assert meta.num_vars == len(var) - 1
+            assert meta.num_mut_vars == len(mut_var) - 2
+            stream = unwrap_stream(var[0])
+            arg_buffer, local_buffer = mut_var[0], mut_var[1]
+            assert isinstance(arg_buffer, GpuBuffer) and isinstance(local_buffer, GpuBuffer)
+            assert arg_buffer.size >= arg_buffer_bytes(meta)
+            length = 0
+            for v in list(var[1:]) + list(mut_var[2:]):
+                if isinstance(v, ScalarArray):
+                    if length in (0, 1):
+                        length = v.len
+                    else:
+                        assert length == v.len
+                elif length == 0:
+                    length = 1
+            assert length > 0
+            arr = (PolyPtr * (meta.num_vars + meta.num_mut_vars))(
+                *[_ptr_of(v) for v in var[1:]], *[_ptr_of(v) for v in mut_var[2:]]
+            )
+            hip.set_device(stream.get_device())
+            hip.check(hip._lib().hipMemcpyAsync(ctypes.c_void_p(arg_buffer.ptr), ctypes.addressof(arr),
+                                                ctypes.sizeof(arr), hip.hipMemcpyHostToDevice, stream.raw()),
+                      "arg buffer H2D")
+            d_vars = arg_buffer.ptr
+            d_mut = arg_buffer.ptr + meta.num_vars * POLYPTR_BYTES
+            hip.check(c_func(d_vars, d_mut, length, True, local_buffer.ptr, stream.raw()), meta.name)
+            # the host array must outlive the async copy
+            stream.sync()
+
+        return Function(FuncMeta(meta.name, KernelType.FusedArith, meta), f)
