@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="headline NTT only (no suite / MSM / batched)")
     ap.add_argument("--msm-only", action="store_true", help="MSM 2^20 leg only (for profiling)")
+    ap.add_argument("--batched-only", action="store_true", help="config-4 batched leg only")
     ap.add_argument("--msm-log-n", type=int, default=20)
     return ap.parse_args()
 
@@ -268,6 +269,53 @@ def msm_leg(libs, stream, log_n, steps, with_cpu):
     return out
 
 
+def batched_leg(libs, stream, steps, with_cpu):
+    """Config 4: 64 x (NTT of log size U{10..22} + fused a*b + gamma*c - z), one hipGraph
+    launch vs the same 64 x (ssip_ntt + fused call) issued one by one."""
+    from zkpoly_amd import fused
+    from zkpoly_amd.batch import Config4Workload
+
+    kernel = fused.FusedKernel(libs, fused.FusedKernelMeta("fused_arith_abgcz", 5, 1, fused.DEMO_DIR))
+    w = Config4Workload(libs, stream, kernel, count=64, lo=10, hi=22, seed=4)
+    graph = w.capture()
+    plain_ms = time_loop(stream, w.run_plain, steps, 2)
+    batched_ms = time_loop(stream, lambda: graph.launch(stream), steps, 2)
+    out = {
+        "instances": 64,
+        "log_sizes": w.log_lens,
+        "elements": w.total_elements,
+        "plain_ms": round(plain_ms, 4),
+        "batched_ms": round(batched_ms, 4),
+        "plain_gelem_s": round(w.total_elements / plain_ms / 1e6, 4),
+        "batched_gelem_s": round(w.total_elements / batched_ms / 1e6, 4),
+        "speedup": round(plain_ms / batched_ms, 4),
+        "launches_plain": 64 * 5,
+        "graph_launches_batched": 1,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        # bit-exact spot check of the smallest and largest instances after one fresh graph launch
+        w.reset_inputs()
+        graph.launch(stream)
+        stream.sync()
+        order = sorted(range(64), key=lambda i: w.lens[i])
+        ok = True
+        for i in (order[0], order[1], order[-1]):
+            x, o = w.results(i)
+            xh, a, b, c = w.host[i]
+            n = w.lens[i]
+            g = np.broadcast_to(w.gamma_z[0], (n, 4)).copy()
+            z = np.broadcast_to(w.gamma_z[1], (n, 4)).copy()
+            want = orc.vec("orc_vec_sub", orc.FR, orc.vec("orc_vec_add", orc.FR, orc.vec("orc_vec_mul", orc.FR, a, b),
+                                                          orc.vec("orc_vec_mul", orc.FR, g, c)), z)
+            ok = ok and np.array_equal(x, orc.ntt(xh, w.log_lens[i], min(16, os.cpu_count() or 1)))
+            ok = ok and np.array_equal(o, want)
+        out["bit_exact_spot_check"] = bool(ok)
+    return out
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -295,6 +343,10 @@ def main():
         sys.exit(1)
     libs = Libs()
     stream = hip.Stream(0)
+    if args.batched_only:
+        log("[bench] batched form only")
+        print(json.dumps({"batched": batched_leg(libs, stream, args.steps, not args.no_cpu)}), flush=True)
+        return
     if args.msm_only:
         log(f"[bench] MSM 2^{args.msm_log_n} only")
         print(json.dumps({"msm": msm_leg(libs, stream, args.msm_log_n, args.steps, not args.no_cpu)}), flush=True)
@@ -371,6 +423,8 @@ def main():
         line["suite"] = suite(libs, stream, max(5, args.steps // 2))
         log("[bench] MSM 2^20 window sweep")
         line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
+        log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
+        line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

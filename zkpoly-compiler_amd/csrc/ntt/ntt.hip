@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../common/poly_ptr.hpp"
 #include "../field/bn254.hpp"
 
@@ -55,21 +57,17 @@ __global__ void __launch_bounds__(THREADS) bitrev_kernel(PolyView v, uint32_t lo
   }
 }
 
-// One pass of DIT stages [s0, s0 + d) over tiles of 2^d rows x 2^c_log columns.
+// One pass of DIT stages [s0, s0 + d) over tile `tile` of 2^d rows x 2^c_log columns.
 //   s0 > 0 : columns are consecutive low indices, rows stride 2^s0
 //            idx(c, k) = lo_block * C + c + (k << s0) + (hi << (s0 + d))
 //   s0 == 0: columns are consecutive contiguous groups of 2^d elements
 //            idx(c, k) = (tile * C + c) << d | k
-__global__ void __launch_bounds__(THREADS)
-    dit_pass(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
-             uint32_t c_log) {
-  __shared__ uint4 lds0[1 << TILE_LOG];
-  __shared__ uint4 lds1[1 << TILE_LOG];
-
+__device__ __forceinline__ void dit_tile(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
+                                         const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+                                         uint32_t c_log, uint64_t tile) {
   const uint32_t C = 1u << c_log;
   const uint32_t R = 1u << d;
   const uint32_t T = R << c_log;
-  const uint64_t tile = blockIdx.x;
 
   uint64_t base, cstride, kstride;
   if (s0 == 0) {
@@ -152,6 +150,77 @@ __global__ void __launch_bounds__(THREADS)
     dst[0] = lds0[pos];
     dst[1] = lds1[pos];
   }
+}
+
+__global__ void __launch_bounds__(THREADS)
+    dit_pass(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+             uint32_t c_log) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+  dit_tile(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- batched (many instances per launch)
+
+// One transform of a batch.
+struct BatchInstance {
+  PolyView v;
+  const uint32_t* tw;
+  uint32_t log_n;
+  uint32_t pad;
+};
+
+// Pass parameters of one instance in one round (d == 0: no pass this round).
+struct BatchPass {
+  uint32_t s0, d, c_log, pad;
+};
+
+// largest k < n with prefix[k] <= g (prefix: n + 1 non-decreasing entries)
+__device__ __forceinline__ uint32_t find_instance(const uint64_t* prefix, uint32_t n, uint64_t g) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Bit reversal of every instance; prefix[k] = sum of earlier instance lengths.
+__global__ void __launch_bounds__(THREADS)
+    bitrev_batched(const BatchInstance* __restrict__ inst, const uint64_t* __restrict__ prefix, uint32_t count) {
+  const uint64_t total = prefix[count];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const uint32_t k = find_instance(prefix, count, g);
+    const uint64_t i = g - prefix[k];
+    const uint32_t log_n = inst[k].log_n;
+    if (log_n == 0) continue;
+    const uint64_t j = __brevll(i) >> (64 - log_n);
+    if (i < j) {
+      const PolyView v = inst[k].v;
+      Fr a = load_elem(v, i);
+      Fr b = load_elem(v, j);
+      store_elem(v, i, b);
+      store_elem(v, j, a);
+    }
+  }
+}
+
+// One pass round over all instances: block b runs tile (b - tile_prefix[k]) of
+// instance k, with tile_prefix[k] = sum of earlier instances' tile counts.
+__global__ void __launch_bounds__(THREADS)
+    dit_pass_batched(const BatchInstance* __restrict__ inst, const BatchPass* __restrict__ pass,
+                     const uint64_t* __restrict__ tile_prefix, uint32_t count) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+  const uint64_t b = blockIdx.x;
+  if (b >= tile_prefix[count]) return;
+  const uint32_t k = find_instance(tile_prefix, count, b);
+  const BatchPass p = pass[k];
+  dit_tile(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k]);
 }
 
 // out[i] = base^i for i < len; each thread starts its chunk with one pow.
@@ -267,6 +336,26 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
   return hipSuccess;
 }
 
+// Device work lists of a batch of transforms (built once, replayed by
+// zkp_ntt_batch_run, which only launches kernels and so can be captured in a
+// hipGraph).
+struct BatchPlan {
+  uint32_t count = 0;
+  BatchInstance* d_inst = nullptr;
+  uint64_t* d_elem_prefix = nullptr;
+  uint64_t total_elems = 0;
+  std::vector<BatchPass*> d_pass;
+  std::vector<uint64_t*> d_tile_prefix;
+  std::vector<uint64_t> total_tiles;
+
+  ~BatchPlan() {
+    if (d_inst) (void)hipFree(d_inst);
+    if (d_elem_prefix) (void)hipFree(d_elem_prefix);
+    for (auto p : d_pass) (void)hipFree(p);
+    for (auto p : d_tile_prefix) (void)hipFree(p);
+  }
+};
+
 }  // namespace ntt
 }  // namespace zkp
 
@@ -366,5 +455,98 @@ hipError_t zkp_gen_powers(unsigned int* out, unsigned long long len, const unsig
                      out, (uint64_t)len, base, chunk);
   return hipGetLastError();
 }
+
+}  // extern "C"
+
+extern "C" {
+
+// Plans `count` independent in-place transforms (xs[i], twiddles[i] device
+// pointers, length 2^log_lens[i]) to run together: one bit-reversal launch and
+// one launch per pass round for the whole batch.  Allocates the device work
+// lists and returns an opaque plan.  No counterpart in the reference ABI.
+hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* twiddles, const unsigned int* log_lens,
+                                unsigned int count, void** plan_out) {
+  if (plan_out == nullptr || (count > 0 && (xs == nullptr || twiddles == nullptr || log_lens == nullptr)))
+    return hipErrorInvalidValue;
+  BatchPlan* plan = new BatchPlan();
+  plan->count = count;
+  std::vector<BatchInstance> inst(count);
+  std::vector<uint64_t> elem_prefix(count + 1, 0);
+  std::vector<std::vector<PassPlan>> plans(count);
+  size_t rounds = 0;
+  for (unsigned i = 0; i < count; i++) {
+    PolyView v;
+    if (!make_view(xs[i], &v) || log_lens[i] > 40 || v.len != (1ull << log_lens[i]) ||
+        (log_lens[i] > 0 && (v.ptr == nullptr || twiddles[i] == nullptr))) {
+      delete plan;
+      return hipErrorInvalidValue;
+    }
+    inst[i] = BatchInstance{v, twiddles[i], log_lens[i], 0};
+    elem_prefix[i + 1] = elem_prefix[i] + (log_lens[i] > 0 ? v.len : 0);
+    if (log_lens[i] > 0) {
+      PassPlan pp[48];
+      const int np = plan_passes(log_lens[i], pp);
+      plans[i].assign(pp, pp + np);
+      if ((size_t)np > rounds) rounds = np;
+    }
+  }
+  plan->total_elems = elem_prefix[count];
+  hipError_t err = hipMalloc(&plan->d_inst, sizeof(BatchInstance) * (count ? count : 1));
+  if (err == hipSuccess) err = hipMalloc(&plan->d_elem_prefix, sizeof(uint64_t) * (count + 1));
+  if (err == hipSuccess && count)
+    err = hipMemcpy(plan->d_inst, inst.data(), sizeof(BatchInstance) * count, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(plan->d_elem_prefix, elem_prefix.data(), sizeof(uint64_t) * (count + 1), hipMemcpyHostToDevice);
+  for (size_t r = 0; r < rounds && err == hipSuccess; r++) {
+    std::vector<BatchPass> passes(count, BatchPass{0, 0, 0, 0});
+    std::vector<uint64_t> tile_prefix(count + 1, 0);
+    for (unsigned i = 0; i < count; i++) {
+      uint64_t tiles = 0;
+      if (r < plans[i].size()) {
+        const PassPlan& q = plans[i][r];
+        passes[i] = BatchPass{q.s0, q.d, q.c_log, 0};
+        tiles = q.tiles;
+      }
+      tile_prefix[i + 1] = tile_prefix[i] + tiles;
+    }
+    BatchPass* dp = nullptr;
+    uint64_t* dt = nullptr;
+    err = hipMalloc(&dp, sizeof(BatchPass) * (count ? count : 1));
+    if (err == hipSuccess) err = hipMalloc(&dt, sizeof(uint64_t) * (count + 1));
+    if (dp) plan->d_pass.push_back(dp);
+    if (dt) plan->d_tile_prefix.push_back(dt);
+    if (err == hipSuccess && count)
+      err = hipMemcpy(dp, passes.data(), sizeof(BatchPass) * count, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = hipMemcpy(dt, tile_prefix.data(), sizeof(uint64_t) * (count + 1), hipMemcpyHostToDevice);
+    plan->total_tiles.push_back(tile_prefix[count]);
+  }
+  if (err != hipSuccess) {
+    delete plan;
+    return err;
+  }
+  *plan_out = plan;
+  return hipSuccess;
+}
+
+// Launches the planned batch on `stream` (kernel launches only).
+hipError_t zkp_ntt_batch_run(void* plan_ptr, hipStream_t stream) {
+  BatchPlan* plan = reinterpret_cast<BatchPlan*>(plan_ptr);
+  if (plan == nullptr) return hipErrorInvalidValue;
+  if (plan->total_elems == 0) return hipSuccess;
+  hipLaunchKernelGGL(bitrev_batched, dim3(grid_for(plan->total_elems, THREADS, 65536)), dim3(THREADS), 0, stream,
+                     plan->d_inst, plan->d_elem_prefix, plan->count);
+  hipError_t err = hipGetLastError();
+  for (size_t r = 0; r < plan->d_pass.size() && err == hipSuccess; r++) {
+    if (plan->total_tiles[r] == 0) continue;
+    hipLaunchKernelGGL(dit_pass_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream, plan->d_inst,
+                       plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
+    err = hipGetLastError();
+  }
+  return err;
+}
+
+// Frees a plan (after the work using it has completed).
+void zkp_ntt_batch_destroy(void* plan_ptr) { delete reinterpret_cast<BatchPlan*>(plan_ptr); }
 
 }  // extern "C"

@@ -66,6 +66,12 @@ def _lib():
         h.hipGetErrorString.restype = ctypes.c_char_p
         h.hipDeviceSynchronize.argtypes = []
         h.hipMemGetInfo.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        h.hipStreamBeginCapture.argtypes = [vp, i]
+        h.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+        h.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, vp, sz]
+        h.hipGraphLaunch.argtypes = [vp, vp]
+        h.hipGraphExecDestroy.argtypes = [vp]
+        h.hipGraphDestroy.argtypes = [vp]
     return _HIP
 
 
@@ -239,5 +245,39 @@ class Event:
         try:
             if self.handle.value:
                 _lib().hipEventDestroy(self.handle)
+        except Exception:
+            pass
+
+
+hipStreamCaptureModeThreadLocal = 1
+
+
+class Graph:
+    """A captured and instantiated hipGraph of the work enqueued by `fill(stream)`.
+
+    Only kernel launches may be captured here (no memsets or allocations).
+    """
+
+    def __init__(self, stream: Stream, fill):
+        h = _lib()
+        self.graph = ctypes.c_void_p(0)
+        self.exec = ctypes.c_void_p(0)
+        check(h.hipStreamBeginCapture(stream.raw(), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")
+        try:
+            fill(stream)
+        finally:
+            rc = h.hipStreamEndCapture(stream.raw(), ctypes.byref(self.graph))
+        check(rc, "hipStreamEndCapture")
+        check(h.hipGraphInstantiate(ctypes.byref(self.exec), self.graph, None, None, 0), "hipGraphInstantiate")
+
+    def launch(self, stream: Stream) -> None:
+        check(_lib().hipGraphLaunch(self.exec, stream.raw()), "hipGraphLaunch")
+
+    def __del__(self):
+        try:
+            if self.exec.value:
+                _lib().hipGraphExecDestroy(self.exec)
+            if self.graph.value:
+                _lib().hipGraphDestroy(self.graph)
         except Exception:
             pass
