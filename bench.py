@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline NTT only (no suite / MSM / batched)")
     ap.add_argument("--msm-only", action="store_true", help="MSM 2^20 leg only (for profiling)")
     ap.add_argument("--batched-only", action="store_true", help="config-4 batched leg only")
+    ap.add_argument("--serving-only", action="store_true", help="config-5 serving leg only")
+    ap.add_argument("--serve-count", type=int, default=60, help="requests per rate in the serving leg")
     ap.add_argument("--msm-log-n", type=int, default=20)
     return ap.parse_args()
 
@@ -316,6 +318,44 @@ def batched_leg(libs, stream, steps, with_cpu):
     return out
 
 
+def serving_leg(libs, stream, rates, count, with_cpu):
+    """Config 5: Poisson streams (70 % NTT 2^14..2^22, 30 % MSM 2^16..2^21) through the
+    continuous-batching engine; p50 / p99 latency and aggregate elements/s per rate."""
+    from zkpoly_amd.serve import Server, Trace, latency_stats, make_resident
+
+    res = make_resident(libs, stream, 14, 22, 21, seed=55)
+    srv = Server(libs, res, slots=4, hbm_budget=16 << 30, msm_window=16)
+    out = {"slots": 4, "hbm_budget_gib": 16, "msm_window": 16, "rates": {}}
+    all_ok = True
+    want = {}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        from zkpoly_amd.field import from_mont
+    for rate in rates:
+        trace = Trace(rate=float(rate), count=count, seed=int(rate))
+        reqs = trace.requests()
+        srv.run(reqs)
+        out["rates"][str(rate)] = latency_stats(reqs, trace.jobs)
+        if with_cpu:
+            threads = min(16, os.cpu_count() or 1)
+            for i, j in enumerate(trace.jobs):
+                key = (j.kind, j.log_n, j.input_id, rate)
+                if key not in want:
+                    x = trace.input(key[:3])
+                    if j.kind == 0:
+                        want[key] = orc.ntt(x, j.log_n, threads)
+                    else:
+                        n = 1 << j.log_n
+                        total = from_mont(orc.to_int(orc.inner_product(res.base_scalars[:n], x)))
+                        want[key] = orc.g1_mul_canonical(orc.generator(), total)
+                all_ok = all_ok and reqs[i].status == 0 and np.array_equal(trace.output(i), want[key])
+            want.clear()
+    if with_cpu:
+        out["bit_exact_all_requests"] = bool(all_ok)
+    return out
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -343,6 +383,12 @@ def main():
         sys.exit(1)
     libs = Libs()
     stream = hip.Stream(0)
+    if args.serving_only:
+        log("[bench] serving form only")
+        line = {"serving": serving_leg(libs, stream, (50, 100, 200, 500, 1000, 2000), args.serve_count,
+                                       not args.no_cpu)}
+        print(json.dumps(line), flush=True)
+        return
     if args.batched_only:
         log("[bench] batched form only")
         print(json.dumps({"batched": batched_leg(libs, stream, args.steps, not args.no_cpu)}), flush=True)
@@ -425,6 +471,9 @@ def main():
         line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
         line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
+        log("[bench] serving form (Poisson 50..2000 req/s)")
+        line["serving"] = serving_leg(libs, stream, (50, 100, 200, 500, 1000, 2000), args.serve_count,
+                                      not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

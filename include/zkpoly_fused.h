@@ -1,0 +1,53 @@
+/* Fused elementwise kernels: the code generator and the generated kernel ABI.
+ *
+ * Generated kernels (one libfused_kernels.so per program, symbol = the
+ * FusedKernelMeta name) have the reference's signature,
+ * core/src/fused_kernels/kernel_gen.rs:286-290 (bound by
+ * core/src/fused_kernels.rs:40-52,97-196):
+ *
+ *   hipError_t <name>(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
+ *                     unsigned long long len, bool is_first,
+ *                     void* local_buffer, hipStream_t stream);
+ *
+ * vars / mut_vars are DEVICE arrays; scalars are 1-element views.  Each
+ * library also exports <name>_batched (instance k reads vars[k*NV..],
+ * mut_vars[k*NM..] and elements [prefix[k], prefix[k+1])) and <name>_arity.
+ */
+#ifndef ZKPOLY_FUSED_H
+#define ZKPOLY_FUSED_H
+
+#include <stdbool.h>
+
+#include "zkpoly_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef hipError_t (*zkp_fused_fn)(const ConstPolyPtr* vars, const PolyPtr* mut_vars, unsigned long long len,
+                                   bool is_first, void* local_buffer, hipStream_t stream);
+typedef hipError_t (*zkp_fused_batched_fn)(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
+                                           const unsigned long long* prefix, unsigned int instances,
+                                           unsigned long long total, bool is_first, hipStream_t stream);
+
+/* library: libzkp_fused_codegen.so */
+
+/* HIP source of one kernel from the serde JSON of common::arith::ArithGraph
+ * (common/src/arith.rs:324-331).  Replaces FusedOp::new + FusedOp::gen,
+ * core/src/fused_kernels/kernel_gen.rs:47-191 (called from
+ * compiler/src/transit/type3/lowering/kernel_gen.rs:96-158).  0 on success,
+ * -1 with a message in err. */
+int zkp_fused_codegen(const char* graph_json, const char* name, char* out, size_t out_cap, size_t* out_len,
+                      char* err, size_t err_cap);
+
+/* Generates every kernel into out_dir/<name>.hip and compiles out_dir/libfused_kernels.so
+ * (the reference's xmake build of the generated sources, core/src/fused_kernels.rs:76-95). */
+int zkp_fused_build(const char* const* graph_jsons, const char* const* names, int count, const char* out_dir,
+                    const char* hipcc, const char* include_dir, const char* const* extra_flags, int n_extra,
+                    char* err, size_t err_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKPOLY_FUSED_H */

@@ -42,6 +42,18 @@ hipError_t zkp_gen_powers(unsigned int* out, unsigned long long len, const unsig
 hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
                                hipEvent_t* events, int max_events, int* kinds, int* num_kernels);
 
+/* Plans `count` independent in-place transforms (device xs[i], twiddles[i],
+ * length 2^log_lens[i]) to run as one bit-reversal launch plus one launch per
+ * pass round for the whole batch; allocates the device work lists. */
+hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* twiddles, const unsigned int* log_lens,
+                                unsigned int count, void** plan_out);
+
+/* Launches a planned batch on `stream` (kernel launches only: capturable in a hipGraph). */
+hipError_t zkp_ntt_batch_run(void* plan, hipStream_t stream);
+
+/* Frees a plan after the work using it has completed. */
+void zkp_ntt_batch_destroy(void* plan);
+
 /* dst[i] = src[i] for n16 16-byte words: plain copy used to calibrate HBM counters. */
 hipError_t zkp_copy_u4(void* dst, const void* src, unsigned long long n16, hipStream_t stream);
 

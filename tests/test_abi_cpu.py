@@ -49,7 +49,8 @@ def exported(path):
 
 def test_headers_declare_something():
     secs = header_sections()
-    assert {s[1] for s in secs} >= {"libntt.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so"}
+    assert {s[1] for s in secs} >= {"libntt.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so",
+                                    "libzkp_fused_codegen.so", "libzkp_serve.so"}
     assert all(s[2] for s in secs)
 
 

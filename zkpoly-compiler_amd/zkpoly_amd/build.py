@@ -131,6 +131,19 @@ def build_fused_demo() -> str:
     return fused.build_library(Libs(), fused.demo_kernels(), fused.DEMO_DIR)
 
 
+def build_serve() -> str:
+    """Host-only serving engine; links libntt.so and libzkp_msm.so."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libzkp_serve.so")
+    tmp = out + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run([shutil.which("g++") or "g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          f"-I{rocm}/include", os.path.join(CSRC, "serve", "serve.cpp"), f"-L{LIB}", "-lntt", "-lzkp_msm",
+          f"-L{rocm}/lib", "-lamdhip64", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_intpeak() -> str:
     os.makedirs(os.path.join(PACKAGE_ROOT, "build"), exist_ok=True)
     out = os.path.join(PACKAGE_ROOT, "build", "intpeak")
@@ -151,6 +164,7 @@ def build_all(jobs: int | None = None) -> list[str]:
         # demo library needs the generator
         futs = [ex.submit(build_msm_abi, w, t, False) for (w, t) in PREBUILT_MSM_CONFIGS]
         futs.append(ex.submit(build_fused_demo))
+        futs.append(ex.submit(build_serve))
         out += [f.result() for f in futs]
     return out
 
