@@ -37,7 +37,7 @@ hipError_t distribute_powers(PolyPtr poly, const unsigned int* powers, unsigned 
 hipError_t zkp_gen_powers(unsigned int* out, unsigned long long len, const unsigned int* base, hipStream_t stream);
 
 /* ssip_ntt that also records events[0] before the first kernel and
- * events[i + 1] after kernel i; kinds[i] = 0 (bit reversal) or 1 (butterfly
+ * events[i + 1] after kernel i; kinds[i] = 0 (bit reversal, fused with the first stages by default) or 1 (butterfly
  * pass); *num_kernels = kernels launched.  Measurement hook. */
 hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned int log_len, hipStream_t stream,
                                hipEvent_t* events, int max_events, int* kinds, int* num_kernels);
@@ -53,6 +53,18 @@ hipError_t zkp_ntt_batch_run(void* plan, hipStream_t stream);
 
 /* Frees a plan after the work using it has completed. */
 void zkp_ntt_batch_destroy(void* plan);
+
+/* Transform variant for later calls and batch plans in this process: 0 = bit
+ * reversal fused with the first stages + radix-4 passes (default), 1 = plain
+ * bit reversal + radix-2 passes.  Returns the previous variant, -1 if unknown. */
+int zkp_ntt_set_variant(int variant);
+
+/* Autotuner hook: passes of 2^log_n transforms read runs of >= 2^min_col_log
+ * adjacent elements (smaller: fewer, deeper passes).  0, or -1 if out of range. */
+int zkp_ntt_set_tuning(unsigned int log_n, unsigned int min_col_log);
+
+/* Kernels one transform of 2^log_n launches with the current settings. */
+int zkp_ntt_kernel_count(unsigned int log_n);
 
 /* dst[i] = src[i] for n16 16-byte words: plain copy used to calibrate HBM counters. */
 hipError_t zkp_copy_u4(void* dst, const void* src, unsigned long long n16, hipStream_t stream);

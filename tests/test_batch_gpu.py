@@ -47,6 +47,28 @@ def test_graph_replays_bit_exact(libs, stream, kernel):
             assert np.array_equal(out, want[i][1]), (replay, i)
 
 
+def test_graph_replays_legacy_variant(libs, stream, kernel):
+    """The batch plan of the radix-2 kernels (variant 1) stays bit-exact too."""
+    from zkpoly_amd.ntt import set_variant
+
+    old = set_variant(libs, 1)
+    try:
+        w = Config4Workload(libs, stream, kernel, count=10, lo=1, hi=14, seed=9)
+        graph = w.capture()
+    finally:
+        set_variant(libs, old)
+    want = [expected(w, i) for i in range(len(w.lens))]
+    for replay in range(2):
+        w.reset_inputs()
+        stream.sync()
+        graph.launch(stream)
+        stream.sync()
+        for i in range(len(w.lens)):
+            x, out = w.results(i)
+            assert np.array_equal(x, want[i][0]), (replay, i, w.log_lens[i])
+            assert np.array_equal(out, want[i][1]), (replay, i)
+
+
 def test_plain_path_matches(libs, stream, kernel):
     w = Config4Workload(libs, stream, kernel, count=12, lo=10, hi=18, seed=8)
     w.run_plain()

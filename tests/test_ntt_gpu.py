@@ -15,7 +15,8 @@ import oracle as orc
 from helpers import cpu2gpu, device_array, download, gpu2cpu, upload
 from zkpoly_amd import hip
 from zkpoly_amd.field import FR_MODULUS, mont_limbs, omega, random_elements, to_mont
-from zkpoly_amd.ntt import DistributePowers, SsipNtt, SsipPrecompute, gen_powers
+from zkpoly_amd.ntt import (DistributePowers, SsipNtt, SsipPrecompute, gen_powers, kernel_count, set_tuning,
+                            set_variant)
 from zkpoly_amd.runtime import ScalarArray
 
 pytestmark = pytest.mark.gpu
@@ -64,6 +65,45 @@ def test_ssip_ntt_random_omega_rotated_slice(libs, stream, k):
     got, _ = gpu_ntt(libs, stream, data, w, rotate=3, sliced=True)
     want = orc.best_fft(data, w, k, THREADS)
     assert np.array_equal(got, want), f"k={k}"
+
+
+DEFAULT_MIN_COL_LOG = 2
+
+
+@pytest.mark.parametrize("variant,min_col", [(0, 1), (0, 3), (0, 6), (1, 2)])
+def test_variants_and_tunings(libs, stream, variant, min_col):
+    """Every kernel plan gives the same best_fft network: the legacy radix-2
+    plan and the fused-head radix-4 plan at several pass depths."""
+    ks = (1, 2, 3, 5, 9, 12, 15, 18)
+    old = set_variant(libs, variant)
+    try:
+        for k in ks:
+            set_tuning(libs, k, min_col)
+            n = 1 << k
+            data = random_elements(n, 300 + k)
+            w = random_elements(1, 900 + k)[0]
+            got, _ = gpu_ntt(libs, stream, data, w, rotate=5 % n, sliced=True)
+            want = orc.best_fft(data, w, k, THREADS)
+            assert np.array_equal(got, want), f"variant={variant} min_col={min_col} k={k}"
+    finally:
+        set_variant(libs, old)
+        for k in ks:
+            set_tuning(libs, k, DEFAULT_MIN_COL_LOG)
+
+
+def test_kernel_counts(libs):
+    assert kernel_count(libs, 0) == 0
+    assert kernel_count(libs, 24) == 4  # fused head + 7 + 6 + 6 stages
+    set_tuning(libs, 24, 1)
+    try:
+        assert kernel_count(libs, 24) == 3  # fused head + 10 + 9 stages
+    finally:
+        set_tuning(libs, 24, DEFAULT_MIN_COL_LOG)
+    old = set_variant(libs, 1)
+    try:
+        assert kernel_count(libs, 24) == 4  # bit reversal + 11 + 9 + 4 stages
+    finally:
+        set_variant(libs, old)
 
 
 @pytest.mark.parametrize("k,rot", [(4, 2), (10, 1023), (13, 5)])

@@ -18,13 +18,19 @@
 // is the caller's [1, w, ..., w^(n/2-1)] (what ssip_precompute writes, the same
 // table the reference's compiler stores as a constant).
 //
-// Kernels:
-//  * bitrev_kernel: in-place swap of i <-> rev(i).
-//  * dit_pass<TILE_LOG>: a workgroup loads a tile of 2^TILE_LOG elements --
-//    2^d butterfly-connected rows times 2^c adjacent columns, so every global
-//    access is a run of >= 4 consecutive 32-byte elements -- into LDS (two
-//    16-byte planes), runs d consecutive stages there, and writes it back.
-//    A transform of 2^24 is bitrev + 3 passes (11 + 9 + 4 stages).
+// Kernels (default plan):
+//  * bitrev_head: the bit-reversal permutation fused with DIT stages 0..4; a
+//    workgroup swaps a pair of 32 x 32 tiles through LDS and runs the five
+//    stages on them before writing back.
+//  * dit_pass_r4: a workgroup loads a tile of 2^TILE_LOG elements -- 2^d
+//    butterfly-connected rows times 2^c adjacent columns, so every global
+//    access is a run of >= 2^c consecutive 32-byte elements -- into LDS (two
+//    16-byte planes), runs d consecutive stages there two at a time (radix-4
+//    register blocks), and writes it back.
+//    A transform of 2^24 is bitrev_head + 3 passes (7 + 6 + 6 stages), or
+//    + 2 passes (10 + 9) with min_col_log = 1.
+// The first version (bitrev_kernel + radix-2 dit_pass, 11 + 9 + 4 stages) is
+// kept selectable through zkp_ntt_set_variant for comparison.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -167,7 +173,7 @@ struct BatchInstance {
   PolyView v;
   const uint32_t* tw;
   uint32_t log_n;
-  uint32_t pad;
+  uint32_t head_t;  // stages fused into the bit reversal (0: none)
 };
 
 // Pass parameters of one instance in one round (d == 0: no pass this round).
@@ -251,6 +257,281 @@ __global__ void __launch_bounds__(THREADS)
   }
 }
 
+// ---------------------------------------------------------------- fused bit reversal + first stages
+//
+// Index i = (a, M, b): a = top t bits, M = middle L - 2t bits, b = low t bits.
+// The bit-reversed array is y[(a, M, b)] = x[(rev b, rev M, rev a)], so the
+// 2^t x 2^t tile of x at middle M' = rev M, read transposed, is the tile of y
+// at middle M.  A block owns the pair {M, rev M}: it loads both x tiles
+// (rows of 2^t contiguous elements), runs DIT stages 0..t-1 (which only mix the
+// low t bits, i.e. stay inside a y row) through the transposed LDS index map,
+// and writes both y tiles back in place.  Rows are padded by one element so
+// the transposed accesses spread over the LDS banks.
+constexpr uint32_t HEAD_T_MAX = 5;
+constexpr uint32_t HEAD_W = 1u << HEAD_T_MAX;
+constexpr uint32_t HEAD_PITCH = HEAD_W + 1;
+
+__device__ __forceinline__ uint32_t rev_bits(uint32_t x, uint32_t bits) { return bits ? __brev(x) >> (32 - bits) : 0; }
+
+__device__ __forceinline__ Fr lds_get(const uint4* l0, const uint4* l1, uint32_t p) {
+  const uint4 a = l0[p], b = l1[p];
+  Fr r;
+  r.v[0] = a.x; r.v[1] = a.y; r.v[2] = a.z; r.v[3] = a.w;
+  r.v[4] = b.x; r.v[5] = b.y; r.v[6] = b.z; r.v[7] = b.w;
+  return r;
+}
+__device__ __forceinline__ void lds_put(uint4* l0, uint4* l1, uint32_t p, const Fr& x) {
+  l0[p] = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  l1[p] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
+}
+
+struct HeadLds {
+  uint4 A0[HEAD_W * HEAD_PITCH], A1[HEAD_W * HEAD_PITCH];
+  uint4 B0[HEAD_W * HEAD_PITCH], B1[HEAD_W * HEAD_PITCH];
+};
+
+__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const uint32_t* __restrict__ tw,
+                                          uint32_t log_n, uint32_t t, uint64_t M) {
+  uint4* A0 = L.A0;
+  uint4* A1 = L.A1;
+  uint4* B0 = L.B0;
+  uint4* B1 = L.B1;
+  const uint32_t W = 1u << t;
+  const uint32_t mbits = log_n - 2 * t;
+  const uint64_t Mr = mbits ? (__brevll(M) >> (64 - mbits)) : 0;
+  if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
+  const bool pair = M != Mr;
+  const uint32_t shift_a = log_n - t;
+  for (uint32_t e = threadIdx.x; e < W * W; e += THREADS) {
+    const uint32_t a = e >> t, b = e & (W - 1);
+    const uint32_t p = a * HEAD_PITCH + b;
+    const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
+    A0[p] = s[0];
+    A1[p] = s[1];
+    if (pair) {
+      const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
+      B0[p] = q[0];
+      B1[p] = q[1];
+    }
+  }
+  __syncthreads();
+  // tile 0 = y at M (stored in the buffer of x at Mr), tile 1 = y at Mr (buffer of x at M)
+  uint4* Y0a = pair ? B0 : A0;
+  uint4* Y0b = pair ? B1 : A1;
+  const uint32_t tiles = pair ? 2 : 1;
+  const uint32_t per_tile = W * (W >> 1);
+  for (uint32_t m = 0; m < t; m++) {
+    const uint32_t half = 1u << m;
+    const uint32_t tw_shift = log_n - 1 - m;
+    for (uint32_t u = threadIdx.x; u < tiles * per_tile; u += THREADS) {
+      const uint32_t tile = u >= per_tile ? 1 : 0;
+      const uint32_t r = u - tile * per_tile;
+      const uint32_t a = r >> (t - 1);
+      const uint32_t q = r & ((W >> 1) - 1);
+      const uint32_t blo = ((q >> m) << (m + 1)) | (q & (half - 1));
+      const uint32_t ra = rev_bits(a, t);
+      const uint32_t pa = rev_bits(blo, t) * HEAD_PITCH + ra;
+      const uint32_t pb = rev_bits(blo + half, t) * HEAD_PITCH + ra;
+      uint4* L0 = tile ? A0 : Y0a;
+      uint4* L1 = tile ? A1 : Y0b;
+      const Fr w = Fr::load(tw + (((uint64_t)(blo & (half - 1)) << tw_shift) << 3));
+      const Fr x = lds_get(L0, L1, pa);
+      const Fr y = lds_get(L0, L1, pb);
+      const Fr tt = y * w;
+      lds_put(L0, L1, pa, x + tt);
+      lds_put(L0, L1, pb, x - tt);
+    }
+    __syncthreads();
+  }
+  for (uint32_t e = threadIdx.x; e < W * W; e += THREADS) {
+    const uint32_t a = e >> t, b = e & (W - 1);
+    const uint32_t p = rev_bits(b, t) * HEAD_PITCH + rev_bits(a, t);
+    uint4* d = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
+    d[0] = Y0a[p];
+    d[1] = Y0b[p];
+    if (pair) {
+      uint4* d2 = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
+      d2[0] = A0[p];
+      d2[1] = A1[p];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(THREADS)
+    bitrev_head(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t) {
+  __shared__ HeadLds L;
+  head_tile(L, v, tw, log_n, t, blockIdx.x);
+}
+
+// Fused bit reversal + first head_t stages of every instance; instance k owns
+// blocks [head_prefix[k], head_prefix[k + 1]) (2^(log_n - 2 head_t) of them, or
+// none when head_t == 0).
+__global__ void __launch_bounds__(THREADS)
+    bitrev_head_batched(const BatchInstance* __restrict__ inst, const uint64_t* __restrict__ head_prefix,
+                        uint32_t count) {
+  __shared__ HeadLds L;
+  const uint64_t b = blockIdx.x;
+  if (b >= head_prefix[count]) return;
+  const uint32_t k = find_instance(head_prefix, count, b);
+  head_tile(L, inst[k].v, inst[k].tw, inst[k].log_n, inst[k].head_t, b - head_prefix[k]);
+}
+
+// ---------------------------------------------------------------- radix-4 pass
+//
+// Same tile geometry as dit_tile, but stages are taken two at a time: a thread
+// owns quads of rows {k0, k0+h, k0+2h, k0+3h} (h = 2^m, bits m and m+1 of k0
+// clear), which stages m and m+1 connect only among themselves.  Stage m uses
+// one twiddle for both of its butterflies (k0 and k0+2h agree modulo 2^(s0+m)),
+// stage m+1 two.  Per LDS round trip a thread does 2 quads = 8 butterflies
+// with 4 independent products per stage; an odd last stage runs radix-2.
+__device__ __forceinline__ uint64_t tile_index(uint64_t base, uint64_t cstride, uint64_t kstride, uint32_t c,
+                                               uint32_t k) {
+  return base + c * cstride + (uint64_t)k * kstride;
+}
+
+__device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
+                                            const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+                                            uint32_t c_log, uint64_t tile) {
+  const uint32_t C = 1u << c_log;
+  const uint32_t R = 1u << d;
+  const uint32_t T = R << c_log;
+  uint64_t base, cstride, kstride;
+  if (s0 == 0) {
+    base = (tile << c_log) << d;
+    cstride = R;
+    kstride = 1;
+  } else {
+    const uint64_t lo_blocks = (1ull << s0) >> c_log;
+    const uint64_t hi = tile / lo_blocks;
+    const uint64_t lo_block = tile % lo_blocks;
+    base = lo_block * C + (hi << (s0 + d));
+    cstride = 1;
+    kstride = 1ull << s0;
+  }
+  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    uint32_t c, k;
+    if (cstride == 1) {
+      c = e & (C - 1);
+      k = e >> c_log;
+    } else {
+      k = e & (R - 1);
+      c = e >> d;
+    }
+    const uint4* src = reinterpret_cast<const uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
+    const uint32_t pos = (k << c_log) | c;
+    lds0[pos] = src[0];
+    lds1[pos] = src[1];
+  }
+  __syncthreads();
+
+  uint32_t m = 0;
+  for (; m + 1 < d; m += 2) {
+    const uint32_t s = s0 + m;
+    const uint32_t h = 1u << m;
+    const uint64_t mask1 = (1ull << s) - 1, mask2 = (1ull << (s + 1)) - 1;
+    const uint32_t sh1 = log_n - 1 - s, sh2 = log_n - 2 - s;
+    const uint32_t quads = T >> 2;
+    for (uint32_t g0 = threadIdx.x; g0 < quads; g0 += 2 * THREADS) {
+      Fr x[2][4], w1[2], w2[2], w3[2];
+      uint32_t pos[2][4];
+      bool live[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const uint32_t g = g0 + u * THREADS;
+        live[u] = g < quads;
+        const uint32_t gg = live[u] ? g : g0;
+        const uint32_t c = gg & (C - 1);
+        const uint32_t qr = gg >> c_log;
+        const uint32_t k0 = ((qr >> m) << (m + 2)) | (qr & (h - 1));
+        const uint64_t i0 = tile_index(base, cstride, kstride, c, k0);
+        const uint64_t i1 = tile_index(base, cstride, kstride, c, k0 + h);
+        w1[u] = Fr::load(tw + (((i0 & mask1) << sh1) << 3));
+        w2[u] = Fr::load(tw + (((i0 & mask2) << sh2) << 3));
+        w3[u] = Fr::load(tw + (((i1 & mask2) << sh2) << 3));
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          pos[u][r] = ((k0 + r * h) << c_log) | c;
+          x[u][r] = lds_get(lds0, lds1, pos[u][r]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const Fr t1 = x[u][1] * w1[u];
+        const Fr t3 = x[u][3] * w1[u];
+        const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
+        const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
+        const Fr t2 = a2 * w2[u];
+        const Fr t4 = a3 * w3[u];
+        x[u][0] = a0 + t2;
+        x[u][2] = a0 - t2;
+        x[u][1] = a1 + t4;
+        x[u][3] = a1 - t4;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!live[u]) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) lds_put(lds0, lds1, pos[u][r], x[u][r]);
+      }
+    }
+    __syncthreads();
+  }
+  if (m < d) {  // odd last stage, radix-2
+    const uint32_t s = s0 + m;
+    const uint32_t half = 1u << m;
+    const uint64_t jmask = (1ull << s) - 1;
+    const uint32_t tw_shift = log_n - 1 - s;
+    for (uint32_t b = threadIdx.x; b < (T >> 1); b += THREADS) {
+      const uint32_t c = b & (C - 1);
+      const uint32_t q = b >> c_log;
+      const uint32_t ka = ((q >> m) << (m + 1)) | (q & (half - 1));
+      const uint32_t pa = (ka << c_log) | c;
+      const uint32_t pb = ((ka + half) << c_log) | c;
+      const Fr w = Fr::load(tw + (((tile_index(base, cstride, kstride, c, ka) & jmask) << tw_shift) << 3));
+      const Fr x = lds_get(lds0, lds1, pa);
+      const Fr y = lds_get(lds0, lds1, pb);
+      const Fr t = y * w;
+      lds_put(lds0, lds1, pa, x + t);
+      lds_put(lds0, lds1, pb, x - t);
+    }
+    __syncthreads();
+  }
+
+  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    uint32_t c, k;
+    if (cstride == 1) {
+      c = e & (C - 1);
+      k = e >> c_log;
+    } else {
+      k = e & (R - 1);
+      c = e >> d;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
+    const uint32_t pos = (k << c_log) | c;
+    dst[0] = lds0[pos];
+    dst[1] = lds1[pos];
+  }
+}
+
+__global__ void __launch_bounds__(THREADS)
+    dit_pass_r4(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+  dit_tile_r4(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(THREADS)
+    dit_pass_r4_batched(const BatchInstance* __restrict__ inst, const BatchPass* __restrict__ pass,
+                        const uint64_t* __restrict__ tile_prefix, uint32_t count) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+  const uint64_t b = blockIdx.x;
+  if (b >= tile_prefix[count]) return;
+  const uint32_t k = find_instance(tile_prefix, count, b);
+  const BatchPass p = pass[k];
+  dit_tile_r4(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k]);
+}
+
 struct PassPlan {
   uint32_t s0, d, c_log;
   uint64_t tiles;
@@ -279,6 +560,45 @@ static int plan_passes(uint32_t log_n, PassPlan* out) {
   return np;
 }
 
+// v2 plan: a fused bit-reversal + first t stages (t = min(5, log_n / 2)), then
+// radix-4 passes of at most TILE_LOG - min_col_log stages each, balanced, with
+// as many adjacent columns as the tile holds.
+struct PlanV2 {
+  uint32_t head_t;  // 0: plain bit reversal, no fused stages
+  int np;
+  PassPlan pass[48];
+};
+
+// per log_n minimum column exponent for v2 passes (runs of >= 2^c elements);
+// set by the autotuner through zkp_ntt_set_tuning
+static uint32_t g_min_col_log[41] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
+                                     2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+static int g_variant = 0;  // 0: v2 (head + radix-4), 1: v1 (bit reversal + radix-2 passes)
+
+static void plan_v2(uint32_t log_n, PlanV2* p) {
+  uint32_t t = log_n / 2;
+  if (t > HEAD_T_MAX) t = HEAD_T_MAX;
+  p->head_t = t;
+  p->np = 0;
+  uint32_t s0 = t;
+  uint32_t min_c = g_min_col_log[log_n < 41 ? log_n : 40];
+  if (min_c > TILE_LOG - 1) min_c = TILE_LOG - 1;
+  const uint32_t max_d = TILE_LOG - min_c;
+  while (s0 < log_n) {
+    const uint32_t rem = log_n - s0;
+    const uint32_t parts = (rem + max_d - 1) / max_d;
+    const uint32_t d = (rem + parts - 1) / parts;
+    uint32_t c = TILE_LOG - d;
+    if (s0 == 0) {
+      if (c > log_n - d) c = log_n - d;
+    } else if (c > s0) {
+      c = s0;
+    }
+    p->pass[p->np++] = PassPlan{s0, d, c, (1ull << log_n) >> (d + c)};
+    s0 += d;
+  }
+}
+
 static inline unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
   uint64_t g = (work + threads - 1) / threads;
   if (g > cap) g = cap;
@@ -304,32 +624,43 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
   if (v.ptr == nullptr || twiddle == nullptr) return hipErrorInvalidValue;
   int ev = 0, nk = 0;
   hipError_t err;
-  if (events && ev < max_events) {
-    err = hipEventRecord(events[ev++], stream);
-    if (err != hipSuccess) return err;
-  }
-  hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
-                     (uint32_t)log_len);
-  err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  if (kinds && nk < max_events) kinds[nk] = 0;
-  nk++;
-  if (events && ev < max_events) {
-    err = hipEventRecord(events[ev++], stream);
-    if (err != hipSuccess) return err;
-  }
-  PassPlan plan[48];
-  const int np = plan_passes(log_len, plan);
-  for (int p = 0; p < np; p++) {
-    hipLaunchKernelGGL(dit_pass, dim3((unsigned)plan[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
-                       (uint32_t)log_len, plan[p].s0, plan[p].d, plan[p].c_log);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    if (kinds && nk < max_events) kinds[nk] = 1;
+  auto mark = [&](int kind) -> hipError_t {
+    if (kinds && nk < max_events) kinds[nk] = kind;
     nk++;
-    if (events && ev < max_events) {
-      err = hipEventRecord(events[ev++], stream);
-      if (err != hipSuccess) return err;
+    if (events && ev < max_events) return hipEventRecord(events[ev++], stream);
+    return hipSuccess;
+  };
+  if (events && ev < max_events) {
+    err = hipEventRecord(events[ev++], stream);
+    if (err != hipSuccess) return err;
+  }
+  if (g_variant == 1) {
+    hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
+                       (uint32_t)log_len);
+    if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
+    PassPlan plan[48];
+    const int np = plan_passes(log_len, plan);
+    for (int p = 0; p < np; p++) {
+      hipLaunchKernelGGL(dit_pass, dim3((unsigned)plan[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
+                         (uint32_t)log_len, plan[p].s0, plan[p].d, plan[p].c_log);
+      if ((err = hipGetLastError()) != hipSuccess || (err = mark(1)) != hipSuccess) return err;
+    }
+  } else {
+    PlanV2 plan;
+    plan_v2(log_len, &plan);
+    if (plan.head_t == 0) {
+      hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
+                         (uint32_t)log_len);
+    } else {
+      const uint64_t blocks = 1ull << (log_len - 2 * plan.head_t);
+      hipLaunchKernelGGL(bitrev_head, dim3((unsigned)blocks), dim3(THREADS), 0, stream, v, twiddle, (uint32_t)log_len,
+                         plan.head_t);
+    }
+    if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
+    for (int p = 0; p < plan.np; p++) {
+      hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)plan.pass[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
+                         (uint32_t)log_len, plan.pass[p].s0, plan.pass[p].d, plan.pass[p].c_log);
+      if ((err = hipGetLastError()) != hipSuccess || (err = mark(1)) != hipSuccess) return err;
     }
   }
   if (num_kernels) *num_kernels = nk;
@@ -341,9 +672,11 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
 // hipGraph).
 struct BatchPlan {
   uint32_t count = 0;
+  int variant = 0;
   BatchInstance* d_inst = nullptr;
-  uint64_t* d_elem_prefix = nullptr;
+  uint64_t* d_elem_prefix = nullptr;  // variant 1: elements; variant 0: head blocks
   uint64_t total_elems = 0;
+  uint64_t total_head_blocks = 0;
   std::vector<BatchPass*> d_pass;
   std::vector<uint64_t*> d_tile_prefix;
   std::vector<uint64_t> total_tiles;
@@ -385,6 +718,41 @@ hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned 
   hipError_t err = zkp::ntt::launch_ntt(x, twiddle, log_len, stream, events, max_events, kinds, &nk);
   if (num_kernels) *num_kernels = nk;
   return err;
+}
+
+// Transform variant for all later calls and batch plans in this process:
+// 0 = fused bit reversal + radix-4 passes (default), 1 = separate bit reversal
+// + radix-2 passes.  Returns the previous variant, or -1 for an unknown one.
+// Not thread-safe against concurrent launches.  No counterpart in the
+// reference ABI (tuning hook).
+int zkp_ntt_set_variant(int variant) {
+  if (variant != 0 && variant != 1) return -1;
+  const int old = zkp::ntt::g_variant;
+  zkp::ntt::g_variant = variant;
+  return old;
+}
+
+// Tuning of transforms of 2^log_n (variant 0): every pass reads runs of at
+// least 2^min_col_log adjacent elements, so a smaller value means fewer,
+// deeper passes.  Returns 0, or -1 when out of range.  No counterpart in the
+// reference ABI (autotuner hook).
+int zkp_ntt_set_tuning(unsigned int log_n, unsigned int min_col_log) {
+  if (log_n > 40 || min_col_log > (unsigned)zkp::ntt::TILE_LOG - 1) return -1;
+  zkp::ntt::g_min_col_log[log_n] = min_col_log;
+  return 0;
+}
+
+// Number of kernels one transform of 2^log_n launches with the current
+// variant and tuning (0 for log_n == 0 or > 40).
+int zkp_ntt_kernel_count(unsigned int log_n) {
+  if (log_n == 0 || log_n > 40) return 0;
+  if (zkp::ntt::g_variant == 1) {
+    zkp::ntt::PassPlan plan[48];
+    return 1 + zkp::ntt::plan_passes(log_n, plan);
+  }
+  zkp::ntt::PlanV2 p;
+  zkp::ntt::plan_v2(log_n, &p);
+  return 1 + p.np;
 }
 
 // dst[i] = src[i] for n 16-byte words: a plain streaming copy used to calibrate
@@ -470,6 +838,7 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
     return hipErrorInvalidValue;
   BatchPlan* plan = new BatchPlan();
   plan->count = count;
+  plan->variant = g_variant;
   std::vector<BatchInstance> inst(count);
   std::vector<uint64_t> elem_prefix(count + 1, 0);
   std::vector<std::vector<PassPlan>> plans(count);
@@ -482,15 +851,29 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
       return hipErrorInvalidValue;
     }
     inst[i] = BatchInstance{v, twiddles[i], log_lens[i], 0};
-    elem_prefix[i + 1] = elem_prefix[i] + (log_lens[i] > 0 ? v.len : 0);
-    if (log_lens[i] > 0) {
+    if (log_lens[i] == 0) {
+      elem_prefix[i + 1] = elem_prefix[i];
+      continue;
+    }
+    if (plan->variant == 1) {
+      elem_prefix[i + 1] = elem_prefix[i] + v.len;
       PassPlan pp[48];
       const int np = plan_passes(log_lens[i], pp);
       plans[i].assign(pp, pp + np);
-      if ((size_t)np > rounds) rounds = np;
+    } else {
+      PlanV2 p2;
+      plan_v2(log_lens[i], &p2);
+      inst[i].head_t = p2.head_t;
+      // head_t == 0 only for log_n == 1, whose bit reversal is the identity
+      elem_prefix[i + 1] = elem_prefix[i] + (p2.head_t ? (1ull << (log_lens[i] - 2 * p2.head_t)) : 0);
+      plans[i].assign(p2.pass, p2.pass + p2.np);
     }
+    if (plans[i].size() > rounds) rounds = plans[i].size();
   }
-  plan->total_elems = elem_prefix[count];
+  if (plan->variant == 1)
+    plan->total_elems = elem_prefix[count];
+  else
+    plan->total_head_blocks = elem_prefix[count];
   hipError_t err = hipMalloc(&plan->d_inst, sizeof(BatchInstance) * (count ? count : 1));
   if (err == hipSuccess) err = hipMalloc(&plan->d_elem_prefix, sizeof(uint64_t) * (count + 1));
   if (err == hipSuccess && count)
@@ -533,14 +916,25 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
 hipError_t zkp_ntt_batch_run(void* plan_ptr, hipStream_t stream) {
   BatchPlan* plan = reinterpret_cast<BatchPlan*>(plan_ptr);
   if (plan == nullptr) return hipErrorInvalidValue;
-  if (plan->total_elems == 0) return hipSuccess;
-  hipLaunchKernelGGL(bitrev_batched, dim3(grid_for(plan->total_elems, THREADS, 65536)), dim3(THREADS), 0, stream,
-                     plan->d_inst, plan->d_elem_prefix, plan->count);
-  hipError_t err = hipGetLastError();
+  hipError_t err = hipSuccess;
+  if (plan->variant == 1) {
+    if (plan->total_elems == 0) return hipSuccess;
+    hipLaunchKernelGGL(bitrev_batched, dim3(grid_for(plan->total_elems, THREADS, 65536)), dim3(THREADS), 0, stream,
+                       plan->d_inst, plan->d_elem_prefix, plan->count);
+    err = hipGetLastError();
+  } else if (plan->total_head_blocks > 0) {
+    hipLaunchKernelGGL(bitrev_head_batched, dim3((unsigned)plan->total_head_blocks), dim3(THREADS), 0, stream,
+                       plan->d_inst, plan->d_elem_prefix, plan->count);
+    err = hipGetLastError();
+  }
   for (size_t r = 0; r < plan->d_pass.size() && err == hipSuccess; r++) {
     if (plan->total_tiles[r] == 0) continue;
-    hipLaunchKernelGGL(dit_pass_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream, plan->d_inst,
-                       plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
+    if (plan->variant == 1)
+      hipLaunchKernelGGL(dit_pass_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream,
+                         plan->d_inst, plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
+    else
+      hipLaunchKernelGGL(dit_pass_r4_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream,
+                         plan->d_inst, plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
     err = hipGetLastError();
   }
   return err;

@@ -44,6 +44,12 @@ def load_lib(libs: Libs) -> ctypes.CDLL:
         lib.distribute_powers.restype = ctypes.c_int
         lib.zkp_gen_powers.argtypes = [u32p, ctypes.c_ulonglong, u32p, ctypes.c_void_p]
         lib.zkp_gen_powers.restype = ctypes.c_int
+        lib.zkp_ntt_set_variant.argtypes = [ctypes.c_int]
+        lib.zkp_ntt_set_variant.restype = ctypes.c_int
+        lib.zkp_ntt_set_tuning.argtypes = [ctypes.c_uint, ctypes.c_uint]
+        lib.zkp_ntt_set_tuning.restype = ctypes.c_int
+        lib.zkp_ntt_kernel_count.argtypes = [ctypes.c_uint]
+        lib.zkp_ntt_kernel_count.restype = ctypes.c_int
         lib._zkp_typed = True
     return lib
 
@@ -113,3 +119,25 @@ class SsipPrecompute:
 def gen_powers(libs: Libs, out_dev: int, length: int, base_dev: int, stream: hip.Stream) -> None:
     """[1, b, b^2, ...] written to device memory (no reference counterpart)."""
     hip.check(load_lib(libs).zkp_gen_powers(out_dev, length, base_dev, stream.raw()), "zkp_gen_powers")
+
+
+VARIANT_FUSED_RADIX4 = 0
+VARIANT_RADIX2 = 1
+
+
+def set_variant(libs: Libs, variant: int) -> int:
+    """Selects the transform kernels for later calls; returns the previous variant."""
+    old = load_lib(libs).zkp_ntt_set_variant(variant)
+    if old < 0:
+        raise ValueError(f"unknown NTT variant {variant}")
+    return old
+
+
+def set_tuning(libs: Libs, log_n: int, min_col_log: int) -> None:
+    """Passes of 2^log_n transforms read runs of >= 2^min_col_log adjacent elements."""
+    if load_lib(libs).zkp_ntt_set_tuning(log_n, min_col_log) != 0:
+        raise ValueError(f"tuning out of range: log_n={log_n} min_col_log={min_col_log}")
+
+
+def kernel_count(libs: Libs, log_n: int) -> int:
+    return int(load_lib(libs).zkp_ntt_kernel_count(log_n))
