@@ -274,11 +274,14 @@ def msm_leg(libs, stream, log_n, steps, with_cpu):
 def batched_leg(libs, stream, steps, with_cpu):
     """Config 4: 64 x (NTT of log size U{10..22} + fused a*b + gamma*c - z), one hipGraph
     launch vs the same 64 x (ssip_ntt + fused call) issued one by one."""
-    from zkpoly_amd import fused
+    from zkpoly_amd import autotune, fused
     from zkpoly_amd.batch import Config4Workload
+    from zkpoly_amd.ntt import kernel_count
 
     kernel = fused.FusedKernel(libs, fused.FusedKernelMeta("fused_arith_abgcz", 5, 1, fused.DEMO_DIR))
-    w = Config4Workload(libs, stream, kernel, count=64, lo=10, hi=22, seed=4)
+    log(f"[bench] autotuning NTT pass depth for log sizes 10..22")
+    tuning = autotune.autotune(libs, stream, ntt_sizes=range(10, 23))
+    w = Config4Workload(libs, stream, kernel, count=64, lo=10, hi=22, seed=4, tuning=tuning)
     graph = w.capture()
     plain_ms = time_loop(stream, w.run_plain, steps, 2)
     batched_ms = time_loop(stream, lambda: graph.launch(stream), steps, 2)
@@ -291,8 +294,9 @@ def batched_leg(libs, stream, steps, with_cpu):
         "plain_gelem_s": round(w.total_elements / plain_ms / 1e6, 4),
         "batched_gelem_s": round(w.total_elements / batched_ms / 1e6, 4),
         "speedup": round(plain_ms / batched_ms, 4),
-        "launches_plain": 64 * 5,
+        "launches_plain": sum(kernel_count(libs, k) + 1 for k in w.log_lens),
         "graph_launches_batched": 1,
+        "autotune_ntt_min_col_log": {str(k): v for k, v in sorted(tuning.ntt.items())},
     }
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -30,6 +30,21 @@ hipError_t ssip_precompute(unsigned int* twiddle, unsigned int len, const unsign
 hipError_t distribute_powers(PolyPtr poly, const unsigned int* powers, unsigned long long power_num,
                              hipStream_t stream);
 
+/* In-place forward NTT of x whose twiddles are recomputed from pq
+ * (2^(pq_deg - 1) entries) and the 32-entry ladder omegas (device arrays from
+ * gen_pq_omegas); no n/2 table.  Bit-identical to ssip_ntt with the table of
+ * omegas[0].  Replaces recompute_ntt, core/src/ntt/src/ntt.h:12 (bound by
+ * core/src/ntt.rs:249-289).  log_len <= 32. */
+hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq, unsigned int pq_deg, const unsigned int* omegas,
+                         unsigned int log_len, hipStream_t stream, const unsigned int max_threads_stage1_log,
+                         const unsigned int max_threads_stage2_log);
+
+/* Host arrays: pq[i] = (unit^(len >> pq_deg))^i for i < 2^(pq_deg - 1), and
+ * omegas[i] = unit^(2^i) for i < 32; synchronous, no GPU work.  Replaces
+ * gen_pq_omegas, core/src/ntt/src/ntt.h:15 (bound by core/src/ntt.rs:337-371). */
+void gen_pq_omegas(unsigned int* pq, unsigned int* omegas, unsigned int pq_deg, unsigned int len,
+                   unsigned int* unit);
+
 /* ---- extensions (no counterpart in the reference ABI) ---- */
 
 /* out[i] = base^i for i < len (device), asynchronous: coset power tables and

@@ -90,6 +90,53 @@ def test_ssip_ntt_rejects_bad_shapes_without_launch():
     assert lib.ssip_ntt(PolyPtr(0x1000, 1, 0, 0, 1), 0x2000, 0, None, 8, 8) == 0
 
 
+def test_recompute_ntt_rejects_bad_args_without_launch():
+    lib = _ntt()
+    ok = PolyPtr(0x1000, 16, 0, 0, 16)
+    assert lib.recompute_ntt(PolyPtr(0x1000, 12, 0, 0, 12), 0x2000, 2, 0x3000, 3, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.recompute_ntt(ok, 0, 2, 0x3000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.recompute_ntt(ok, 0x2000, 2, 0, 4, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.recompute_ntt(ok, 0x2000, 0, 0x3000, 4, None, 8, 8) == HIP_INVALID_VALUE
+    # the omega ladder has 32 entries, so transforms stop at 2^32
+    big = 1 << 33
+    assert lib.recompute_ntt(PolyPtr(0x1000, big, 0, 0, big), 0x2000, 2, 0x3000, 33, None, 8, 8) == HIP_INVALID_VALUE
+    assert lib.recompute_ntt(PolyPtr(0x1000, 1, 0, 0, 1), 0x2000, 1, 0x3000, 0, None, 8, 8) == 0
+
+
+def test_gen_pq_omegas_host_matches_oracle():
+    """gen_pq_omegas is host code: checked here against the oracle's restatement
+    (itself pinned to big-int powers in test_oracle_ntt_msm)."""
+    import numpy as np
+
+    import oracle as orc
+    from zkpoly_amd.field import random_elements
+    from zkpoly_amd.ntt import GenPqOmegas
+
+    gen = GenPqOmegas(Libs()).get_fn()
+    for log_len, seed in ((1, 1), (5, 2), (12, 3), (18, 4), (24, 5)):
+        unit = random_elements(1, 40 + seed)[0]  # not a root of unity: exact powers still agree
+        pq_len = GenPqOmegas.get_pq_len(log_len)
+        pq = np.zeros((pq_len, 4), dtype=np.uint64)
+        om = np.zeros((32, 4), dtype=np.uint64)
+        gen(ScalarArray(pq_len, pq.ctypes.data, device="cpu"), ScalarArray(32, om.ctypes.data, device="cpu"),
+            1 << log_len, unit)
+        want_pq, want_om = orc.gen_pq_omegas(pq_len.bit_length(), 1 << log_len, unit)
+        assert np.array_equal(pq, want_pq), log_len
+        assert np.array_equal(om, want_om), log_len
+
+
+def test_get_pq_len_follows_the_reference_stage_split():
+    """core/src/ntt.rs:322-335 with stage threads (8, 8): worked by hand for
+    2^24 (stage-1 degree 12 -> 2 rounds of 6; stage-2 degree 12 -> 3 rounds of
+    4: 2^(6-1)) and 2^18 (9 fits one round: 2^8); never above 256 entries."""
+    from zkpoly_amd.ntt import GenPqOmegas
+
+    assert GenPqOmegas.get_pq_len(24) == 32
+    assert GenPqOmegas.get_pq_len(18) == 256
+    assert GenPqOmegas.get_pq_len(1) == 1
+    assert max(GenPqOmegas.get_pq_len(k) for k in range(1, 33)) == 256
+
+
 def test_distribute_powers_rejects_bad_args():
     lib = _ntt()
     assert lib.distribute_powers(PolyPtr(0x1000, 16, 0, 0, 16), 0x2000, 0, None) == HIP_INVALID_VALUE

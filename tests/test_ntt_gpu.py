@@ -15,8 +15,8 @@ import oracle as orc
 from helpers import cpu2gpu, device_array, download, gpu2cpu, upload
 from zkpoly_amd import hip
 from zkpoly_amd.field import FR_MODULUS, mont_limbs, omega, random_elements, to_mont
-from zkpoly_amd.ntt import (DistributePowers, SsipNtt, SsipPrecompute, gen_powers, kernel_count, set_tuning,
-                            set_variant)
+from zkpoly_amd.ntt import (DistributePowers, GenPqOmegas, RecomputeNtt, SsipNtt, SsipPrecompute, gen_powers,
+                            kernel_count, set_tuning, set_variant)
 from zkpoly_amd.runtime import ScalarArray
 
 pytestmark = pytest.mark.gpu
@@ -131,6 +131,100 @@ def test_round_trip_2_16(libs, stream):
     dist([x], [ninv, stream])
     stream.sync()
     assert np.array_equal(download(x, buf), data)
+
+
+def pq_omegas(libs, k, w_mont, pq_len=None):
+    """Host pq / omega ladder through gen_pq_omegas, uploaded (as the compiler does)."""
+    pq_len = pq_len or GenPqOmegas.get_pq_len(k)
+    pq = np.zeros((pq_len, 4), dtype=np.uint64)
+    om = np.zeros((32, 4), dtype=np.uint64)
+    GenPqOmegas(libs).get_fn()(ScalarArray(pq_len, pq.ctypes.data, device="cpu"),
+                               ScalarArray(32, om.ctypes.data, device="cpu"), 1 << k, w_mont)
+    pq_buf, pq_arr = upload(pq)
+    om_buf, om_arr = upload(om)
+    return (pq_buf, pq_arr), (om_buf, om_arr)
+
+
+@pytest.mark.parametrize("k", list(range(1, 21)))
+def test_recompute_ntt_random_omega_rotated_slice(libs, stream, k):
+    """core/tests/ntt.rs:151-259 (random omega, pq sized by get_pq_len) on a
+    rotated slice of a 2^(k+1) buffer."""
+    n = 1 << k
+    data = random_elements(n, 2000 + k)
+    w = random_elements(1, 6000 + k)[0]
+    (pqb, pq), (omb, om) = pq_omegas(libs, k, w)
+    buf, whole = device_array(2 * n)
+    whole.rotate(3)
+    x = whole.slice(n, 2 * n)
+    cpu2gpu(data, x, buf)
+    RecomputeNtt(libs).get_fn().f([x], [pq, om, stream], lambda i: i)
+    stream.sync()
+    assert np.array_equal(gpu2cpu(x, buf), orc.best_fft(data, w, k, THREADS)), f"k={k}"
+
+
+@pytest.mark.parametrize("pq_len", [1, 2, 256])
+def test_recompute_ntt_any_pq_size(libs, stream, pq_len):
+    """The table size is the caller's choice: coarse pq (ladder-built small
+    roots) and fine pq give the same transform."""
+    k = 17
+    data = random_elements(1 << k, 71)
+    w = random_elements(1, 72)[0]
+    (pqb, pq), (omb, om) = pq_omegas(libs, k, w, pq_len)
+    buf, x = device_array(1 << k)
+    cpu2gpu(data, x, buf)
+    RecomputeNtt(libs).get_fn().f([x], [pq, om, stream])
+    stream.sync()
+    assert np.array_equal(gpu2cpu(x, buf), orc.best_fft(data, w, k, THREADS))
+
+
+@pytest.mark.parametrize("k", [22, 24])
+def test_recompute_matches_ssip_at_workload_sizes(libs, stream, k):
+    """At 2^22 / 2^24 the recompute transform equals the table transform (which
+    the workload-size test pins to the oracle) byte for byte."""
+    n = 1 << k
+    data = random_elements(n, 8800 + k)
+    w = orc.omega(k)
+    (pqb, pq), (omb, om) = pq_omegas(libs, k, w)
+    got_r = _run_on_device(data, lambda x: RecomputeNtt(libs).get_fn().f([x], [pq, om, stream]), stream)
+    tw_buf, tw = device_array(n // 2)
+    base_buf, _ = upload(w)
+    gen_powers(libs, tw.values, n // 2, base_buf.addr, stream)
+    got_s = _run_on_device(data, lambda x: SsipNtt(libs).get_fn().f([x], [tw, stream]), stream)
+    assert np.array_equal(got_r, got_s)
+    # size-independent checks: linearity against the delta input (all ones)
+    delta = np.zeros((n, 4), dtype=np.uint64)
+    delta[0] = mont_limbs(1)
+    ones = _run_on_device(delta, lambda x: RecomputeNtt(libs).get_fn().f([x], [pq, om, stream]), stream)
+    assert np.all(ones == mont_limbs(1))
+
+
+def _run_on_device(data, fn, stream):
+    buf, x = device_array(data.shape[0])
+    cpu2gpu(data, x, buf)
+    fn(x)
+    stream.sync()
+    return gpu2cpu(x, buf)
+
+
+def test_distribute_zeta_extended_recompute(libs, stream):
+    """core/tests/ntt.rs:261-388 literally: the reference runs this one through
+    RecomputeNtt with pq / omegas of the extended domain."""
+    k = 12
+    ek = k + 2
+    data = random_elements(1 << k, 32)
+    padded = np.zeros((1 << ek, 4), dtype=np.uint64)
+    padded[: 1 << k] = data
+    zeta = pow(7, (FR_MODULUS - 1) // 3, FR_MODULUS)
+    zetas = np.stack([mont_limbs(1), mont_limbs(zeta), mont_limbs(zeta * zeta % FR_MODULUS)])
+    (pqb, pq), (omb, om) = pq_omegas(libs, ek, orc.omega(ek))
+    buf, x = device_array(1 << ek)
+    cpu2gpu(padded, x, buf)
+    z_buf, z = upload(zetas)
+    DistributePowers(libs).get_fn().f([x], [z, stream])
+    RecomputeNtt(libs).get_fn().f([x], [pq, om, stream])
+    stream.sync()
+    want = orc.ntt(orc.distribute_powers(padded, zetas), ek, THREADS)
+    assert np.array_equal(gpu2cpu(x, buf), want)
 
 
 def test_distribute_zeta_extended(libs, stream):

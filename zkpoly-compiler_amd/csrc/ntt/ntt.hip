@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <vector>
 
 #include "../common/poly_ptr.hpp"
@@ -290,8 +291,28 @@ struct HeadLds {
   uint4 B0[HEAD_W * HEAD_PITCH], B1[HEAD_W * HEAD_PITCH];
 };
 
-__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const uint32_t* __restrict__ tw,
-                                          uint32_t log_n, uint32_t t, uint64_t M) {
+// Twiddle of DIT stage m for j < 2^m: w^(j * 2^(log_n - 1 - m)), read from the
+// caller's table [1, w, ..., w^(n/2 - 1)].
+struct TableTw {
+  const uint32_t* tw;
+  uint32_t log_n;
+  __device__ __forceinline__ Fr get(uint32_t m, uint32_t j) const {
+    return Fr::load(tw + (((uint64_t)j << (log_n - 1 - m)) << 3));
+  }
+};
+
+// The same twiddles from an LDS table r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1),
+// for stages m < t.
+struct LdsTw {
+  const uint4* r0;
+  const uint4* r1;
+  uint32_t t;
+  __device__ __forceinline__ Fr get(uint32_t m, uint32_t j) const { return lds_get(r0, r1, j << (t - 1 - m)); }
+};
+
+template <class TW>
+__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const TW& tws, uint32_t log_n, uint32_t t,
+                                          uint64_t M) {
   uint4* A0 = L.A0;
   uint4* A1 = L.A1;
   uint4* B0 = L.B0;
@@ -322,7 +343,6 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const u
   const uint32_t per_tile = W * (W >> 1);
   for (uint32_t m = 0; m < t; m++) {
     const uint32_t half = 1u << m;
-    const uint32_t tw_shift = log_n - 1 - m;
     for (uint32_t u = threadIdx.x; u < tiles * per_tile; u += THREADS) {
       const uint32_t tile = u >= per_tile ? 1 : 0;
       const uint32_t r = u - tile * per_tile;
@@ -334,7 +354,7 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const u
       const uint32_t pb = rev_bits(blo + half, t) * HEAD_PITCH + ra;
       uint4* L0 = tile ? A0 : Y0a;
       uint4* L1 = tile ? A1 : Y0b;
-      const Fr w = Fr::load(tw + (((uint64_t)(blo & (half - 1)) << tw_shift) << 3));
+      const Fr w = tws.get(m, blo & (half - 1));
       const Fr x = lds_get(L0, L1, pa);
       const Fr y = lds_get(L0, L1, pb);
       const Fr tt = y * w;
@@ -360,7 +380,7 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const u
 __global__ void __launch_bounds__(THREADS)
     bitrev_head(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t) {
   __shared__ HeadLds L;
-  head_tile(L, v, tw, log_n, t, blockIdx.x);
+  head_tile(L, v, TableTw{tw, log_n}, log_n, t, blockIdx.x);
 }
 
 // Fused bit reversal + first head_t stages of every instance; instance k owns
@@ -373,7 +393,7 @@ __global__ void __launch_bounds__(THREADS)
   const uint64_t b = blockIdx.x;
   if (b >= head_prefix[count]) return;
   const uint32_t k = find_instance(head_prefix, count, b);
-  head_tile(L, inst[k].v, inst[k].tw, inst[k].log_n, inst[k].head_t, b - head_prefix[k]);
+  head_tile(L, inst[k].v, TableTw{inst[k].tw, inst[k].log_n}, inst[k].log_n, inst[k].head_t, b - head_prefix[k]);
 }
 
 // ---------------------------------------------------------------- radix-4 pass
@@ -532,6 +552,203 @@ __global__ void __launch_bounds__(THREADS)
   dit_tile_r4(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k]);
 }
 
+// ---------------------------------------------------------------- recompute NTT
+//
+// recompute_ntt (core/src/ntt/src/recompute_ntt.cuh:361-438) runs the same
+// transform without the n/2-entry twiddle table: the caller passes
+// pq[i] = w^(i * 2^(log_n - pq_deg)) for i < 2^(pq_deg - 1) and the ladder
+// omegas[i] = w^(2^i), i < 32 (gen_pq_omegas).  Every twiddle is formed here as
+// an exact product of those powers, so it equals the table entry w^e for the
+// same exponent e and the result is bit-identical to ssip_ntt / best_fft.
+//
+// Twiddle of stage s = s0 + m at global index i (i mod 2^s = lo + kp * 2^s0,
+// lo < 2^s0 the tile column, kp < 2^m the row bits):
+//   w^((i mod 2^s) * 2^(L-1-s)) = colbase(lo, m) * small(kp << (d - 1 - m))
+//   colbase(lo, m) = w^(lo * 2^(L-1-s0-m)),  small(j) = w^(j * 2^(L-d)).
+// A workgroup walks consecutive tiles, so lo grows by C from one tile to the
+// next and colbase is advanced with one product per (column, stage).
+constexpr int RTILE_LOG = 10;          // 32 KiB of tile + 16 KiB of stage twiddles
+constexpr uint32_t RMAX_C_LOG = 5;     // <= 32 columns
+constexpr uint32_t RMAX_D = RTILE_LOG - 1;
+constexpr uint32_t RMAX_COLBASE = 160;  // max C * d over plans with c <= 5, c + d <= 10
+constexpr uint32_t RBLOCKS = 1024;
+
+// w^(e * 2^sh) as the product of omegas[b + sh] over the set bits b of e.
+__device__ __forceinline__ Fr pow_ladder(const uint32_t* __restrict__ omegas, uint64_t e, uint32_t sh) {
+  Fr r = Fr::one();
+  bool first = true;
+  while (e) {
+    const uint32_t b = __ffsll((unsigned long long)e) - 1;
+    e &= e - 1;
+    const Fr f = Fr::load(omegas + (b + sh) * 8);
+    r = first ? f : r * f;
+    first = false;
+  }
+  return r;
+}
+
+// r[j] = w^(j * 2^(log_n - D)) for j < 2^(D - 1): copied out of pq when it is
+// fine enough, else built by doubling from the ladder.  Block-cooperative; ends
+// with a barrier.
+__device__ void build_small_roots(uint4* r0, uint4* r1, const uint32_t* __restrict__ pq, uint32_t pq_deg,
+                                  const uint32_t* __restrict__ omegas, uint32_t log_n, uint32_t D) {
+  const uint32_t cnt = (1u << D) >> 1;
+  if (pq_deg >= D && pq_deg <= log_n) {
+    for (uint32_t j = threadIdx.x; j < cnt; j += THREADS) lds_put(r0, r1, j, Fr::load(pq + ((uint64_t)j << (pq_deg - D)) * 8));
+  } else {
+    if (threadIdx.x == 0) lds_put(r0, r1, 0, Fr::one());
+    for (uint32_t i = 0; (1u << i) < cnt; i++) {
+      __syncthreads();
+      const Fr g = Fr::load(omegas + (log_n - D + i) * 8);
+      for (uint32_t j = threadIdx.x; j < (1u << i); j += THREADS)
+        lds_put(r0, r1, j + (1u << i), lds_get(r0, r1, j) * g);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(THREADS)
+    rbitrev_head(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg, const uint32_t* __restrict__ omegas,
+                 uint32_t log_n, uint32_t t) {
+  __shared__ HeadLds L;
+  __shared__ uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
+  const uint64_t M = blockIdx.x;
+  const uint32_t mbits = log_n - 2 * t;
+  if (mbits && M > (__brevll(M) >> (64 - mbits))) return;  // uniform per block
+  build_small_roots(r0, r1, pq, pq_deg, omegas, log_n, t);
+  head_tile(L, v, LdsTw{r0, r1, t}, log_n, t, M);
+}
+
+struct RPassLds {
+  uint4 x0[1 << RTILE_LOG], x1[1 << RTILE_LOG];              // tile
+  uint4 w0[1 << (RTILE_LOG - 1)], w1[1 << (RTILE_LOG - 1)];  // twiddles of the current stage
+  uint4 r0[1 << (RMAX_D - 1)], r1[1 << (RMAX_D - 1)];        // small roots
+  uint4 b0[RMAX_COLBASE], b1[RMAX_COLBASE];                  // colbase of the current tile, [c * d + m]
+  uint4 z0[RMAX_COLBASE], z1[RMAX_COLBASE];                  // colbase at lo_block 0
+};
+
+// Stages [s0, s0 + d) over tiles [blockIdx.x * per_block, ...) of 2^d rows x
+// 2^c_log columns, tile geometry as dit_tile.
+__global__ void __launch_bounds__(THREADS)
+    rdit_pass(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg, const uint32_t* __restrict__ omegas,
+              uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log, uint64_t tiles, uint32_t per_block) {
+  __shared__ RPassLds S;
+  const uint32_t C = 1u << c_log;
+  const uint32_t R = 1u << d;
+  const uint32_t T = R << c_log;
+  const uint64_t first = (uint64_t)blockIdx.x * per_block;
+  if (first >= tiles) return;
+  const uint64_t last = first + per_block < tiles ? first + per_block : tiles;
+  const uint64_t lo_blocks = s0 ? ((1ull << s0) >> c_log) : 1;
+
+  if (s0) {
+    const uint64_t lob = first % lo_blocks;
+    for (uint32_t e = threadIdx.x; e < C * d; e += THREADS) {
+      const uint32_t c = e / d, m = e - c * d;
+      const uint32_t sh = log_n - 1 - s0 - m;
+      lds_put(S.z0, S.z1, e, pow_ladder(omegas, c, sh));
+      lds_put(S.b0, S.b1, e, pow_ladder(omegas, lob * C + c, sh));
+    }
+  }
+  build_small_roots(S.r0, S.r1, pq, pq_deg, omegas, log_n, d);  // ends with a barrier
+
+  for (uint64_t tile = first; tile < last; tile++) {
+    uint64_t base, cstride, kstride;
+    if (s0 == 0) {
+      base = (tile << c_log) << d;
+      cstride = R;
+      kstride = 1;
+    } else {
+      const uint64_t hi = tile / lo_blocks;
+      const uint64_t lo_block = tile % lo_blocks;
+      base = lo_block * C + (hi << (s0 + d));
+      cstride = 1;
+      kstride = 1ull << s0;
+      if (tile != first) {  // lo grows by C, or wraps to 0 with the next hi
+        const bool wrap = lo_block == 0;
+        for (uint32_t e = threadIdx.x; e < C * d; e += THREADS) {
+          const uint32_t m = e % d;
+          const Fr nb = wrap ? lds_get(S.z0, S.z1, e)
+                             : lds_get(S.b0, S.b1, e) * Fr::load(omegas + (c_log + log_n - 1 - s0 - m) * 8);
+          lds_put(S.b0, S.b1, e, nb);
+        }
+      }
+    }
+    for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+      uint32_t c, k;
+      if (cstride == 1) {
+        c = e & (C - 1);
+        k = e >> c_log;
+      } else {
+        k = e & (R - 1);
+        c = e >> d;
+      }
+      const uint4* src = reinterpret_cast<const uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
+      const uint32_t pos = (k << c_log) | c;
+      S.x0[pos] = src[0];
+      S.x1[pos] = src[1];
+    }
+    __syncthreads();
+    for (uint32_t m = 0; m < d; m++) {
+      const uint32_t h = 1u << m;
+      for (uint32_t e = threadIdx.x; e < (C << m); e += THREADS) {
+        const uint32_t c = e >> m, kp = e & (h - 1);
+        Fr w = lds_get(S.r0, S.r1, kp << (d - 1 - m));
+        if (s0) w = lds_get(S.b0, S.b1, c * d + m) * w;
+        lds_put(S.w0, S.w1, e, w);
+      }
+      __syncthreads();
+      for (uint32_t b0 = threadIdx.x; b0 < (T >> 1); b0 += 2 * THREADS) {
+        Fr x[2], y[2], w[2];
+        uint32_t pa[2], pb[2];
+        bool live[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const uint32_t b = b0 + u * THREADS;
+          live[u] = b < (T >> 1);
+          const uint32_t bb = live[u] ? b : b0;
+          const uint32_t c = bb & (C - 1);
+          const uint32_t q = bb >> c_log;
+          const uint32_t ka = ((q >> m) << (m + 1)) | (q & (h - 1));
+          pa[u] = (ka << c_log) | c;
+          pb[u] = ((ka + h) << c_log) | c;
+          w[u] = lds_get(S.w0, S.w1, (c << m) | (ka & (h - 1)));
+          x[u] = lds_get(S.x0, S.x1, pa[u]);
+          y[u] = lds_get(S.x0, S.x1, pb[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const Fr t = y[u] * w[u];
+          y[u] = x[u] - t;
+          x[u] = x[u] + t;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          if (!live[u]) continue;
+          lds_put(S.x0, S.x1, pa[u], x[u]);
+          lds_put(S.x0, S.x1, pb[u], y[u]);
+        }
+      }
+      __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+      uint32_t c, k;
+      if (cstride == 1) {
+        c = e & (C - 1);
+        k = e >> c_log;
+      } else {
+        k = e & (R - 1);
+        c = e >> d;
+      }
+      uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
+      const uint32_t pos = (k << c_log) | c;
+      dst[0] = S.x0[pos];
+      dst[1] = S.x1[pos];
+    }
+    __syncthreads();
+  }
+}
+
 struct PassPlan {
   uint32_t s0, d, c_log;
   uint64_t tiles;
@@ -589,6 +806,31 @@ static void plan_v2(uint32_t log_n, PlanV2* p) {
     const uint32_t parts = (rem + max_d - 1) / max_d;
     const uint32_t d = (rem + parts - 1) / parts;
     uint32_t c = TILE_LOG - d;
+    if (s0 == 0) {
+      if (c > log_n - d) c = log_n - d;
+    } else if (c > s0) {
+      c = s0;
+    }
+    p->pass[p->np++] = PassPlan{s0, d, c, (1ull << log_n) >> (d + c)};
+    s0 += d;
+  }
+}
+
+// Recompute plan: the fused head (t = min(5, log_n / 2)), then balanced passes
+// of <= RMAX_D stages over tiles of <= 2^RTILE_LOG elements with <= 2^RMAX_C_LOG
+// columns.
+static void plan_recompute(uint32_t log_n, PlanV2* p) {
+  uint32_t t = log_n / 2;
+  if (t > HEAD_T_MAX) t = HEAD_T_MAX;
+  p->head_t = t;
+  p->np = 0;
+  uint32_t s0 = t;
+  while (s0 < log_n) {
+    const uint32_t rem = log_n - s0;
+    const uint32_t parts = (rem + RMAX_D - 1) / RMAX_D;
+    const uint32_t d = (rem + parts - 1) / parts;
+    uint32_t c = RTILE_LOG - d;
+    if (c > RMAX_C_LOG) c = RMAX_C_LOG;
     if (s0 == 0) {
       if (c > log_n - d) c = log_n - d;
     } else if (c > s0) {
@@ -753,6 +995,71 @@ int zkp_ntt_kernel_count(unsigned int log_n) {
   zkp::ntt::PlanV2 p;
   zkp::ntt::plan_v2(log_n, &p);
   return 1 + p.np;
+}
+
+// In-place natural-order forward NTT of x (len = 2^log_len) whose twiddles are
+// recomputed from pq (2^(pq_deg - 1) entries) and the 32-entry ladder omegas,
+// both device arrays written by gen_pq_omegas.  Replaces recompute_ntt of
+// core/src/ntt/src/ntt.h:12 / recompute_ntt.cuh:361-438 (bound by
+// core/src/ntt.rs:249-289).  Allocates nothing; asynchronous on stream.  The
+// thread-shape hints are accepted but not needed here.
+hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq, unsigned int pq_deg, const unsigned int* omegas,
+                         unsigned int log_len, hipStream_t stream, const unsigned int max_threads_stage1_log,
+                         const unsigned int max_threads_stage2_log) {
+  (void)max_threads_stage1_log;
+  (void)max_threads_stage2_log;
+  using namespace zkp::ntt;
+  PolyView v;
+  if (!make_view(x, &v)) return hipErrorInvalidValue;
+  if (log_len > 32 || v.len != (1ull << log_len)) return hipErrorInvalidValue;
+  if (log_len == 0) return hipSuccess;
+  if (v.ptr == nullptr || pq == nullptr || omegas == nullptr || pq_deg == 0) return hipErrorInvalidValue;
+  PlanV2 plan;
+  plan_recompute(log_len, &plan);
+  if (plan.head_t == 0) {
+    hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
+                       (uint32_t)log_len);
+  } else {
+    hipLaunchKernelGGL(rbitrev_head, dim3((unsigned)(1ull << (log_len - 2 * plan.head_t))), dim3(THREADS), 0, stream,
+                       v, pq, (uint32_t)pq_deg, omegas, (uint32_t)log_len, plan.head_t);
+  }
+  hipError_t err = hipGetLastError();
+  for (int p = 0; p < plan.np && err == hipSuccess; p++) {
+    const PassPlan& q = plan.pass[p];
+    const uint64_t blocks = q.tiles < RBLOCKS ? q.tiles : RBLOCKS;
+    const uint32_t per_block = (uint32_t)((q.tiles + blocks - 1) / blocks);
+    const unsigned grid = (unsigned)((q.tiles + per_block - 1) / per_block);
+    hipLaunchKernelGGL(rdit_pass, dim3(grid), dim3(THREADS), 0, stream, v, pq, (uint32_t)pq_deg, omegas,
+                       (uint32_t)log_len, q.s0, q.d, q.c_log, q.tiles, per_block);
+    err = hipGetLastError();
+  }
+  return err;
+}
+
+// pq[i] = (unit^(len >> pq_deg))^i for i < 2^(pq_deg - 1) (pq[0] = 1 is always
+// written) and omegas[i] = unit^(2^i) for i < 32; host arrays, synchronous.
+// Replaces gen_pq_omegas of core/src/ntt/src/ntt.h:15 / precompute.cuh:88-112
+// (bound by core/src/ntt.rs:337-371).
+void gen_pq_omegas(unsigned int* pq, unsigned int* omegas, unsigned int pq_deg, unsigned int len,
+                   unsigned int* unit) {
+  using zkp::Fr;
+  auto put = [](unsigned int* dst, const Fr& x) { memcpy(dst, x.v, sizeof(x.v)); };
+  const Fr u = Fr::from_raw(unit);
+  const uint64_t cnt = pq_deg < 64 ? (1ull << pq_deg) >> 1 : 0;
+  put(pq, Fr::one());
+  if (cnt > 1) {
+    const Fr step = u.pow(pq_deg < 32 ? (uint64_t)(len >> pq_deg) : 0);
+    Fr cur = step;
+    for (uint64_t i = 1; i < cnt; i++) {
+      put(pq + i * 8, cur);
+      cur = cur * step;
+    }
+  }
+  Fr cur = u;
+  for (int i = 0; i < 32; i++) {
+    put(omegas + i * 8, cur);
+    cur = cur.sqr();
+  }
 }
 
 // dst[i] = src[i] for n 16-byte words: a plain streaming copy used to calibrate

@@ -109,10 +109,11 @@ class ProofStepBatch:
 class Config4Workload:
     """`count` instances: NTT of x_i (log size uniform in [lo, hi], seeded), then
     out_i = a_i * b_i + gamma * c_i - z (seeded vectors of the same size, seeded
-    scalars gamma, z), with the fused kernel `kernel` (vars [a, b, c, gamma, z])."""
+    scalars gamma, z), with the fused kernel `kernel` (vars [a, b, c, gamma, z]).
+    `tuning` (an autotune.Tuning) is installed before the transforms are planned."""
 
     def __init__(self, libs: Libs, stream: hip.Stream, kernel: FusedKernel, count: int = 64, lo: int = 10,
-                 hi: int = 22, seed: int = 4):
+                 hi: int = 22, seed: int = 4, tuning=None):
         from .field import mont_limbs, omega, random_elements
         from .ntt import gen_powers
 
@@ -164,6 +165,8 @@ class Config4Workload:
             dm.upload(np.frombuffer(bytes(ma), dtype=np.uint8))
             self.plain_args.append((dv, dm))
         self.ntt_lib = load_ntt(libs)
+        if tuning is not None:
+            tuning.apply(libs)
         self.ntt_batch = NttBatch(libs, self.x_ptrs, [self.tw[k][0].addr for k in self.log_lens], self.log_lens)
         self.fused_batch = FusedBatch(kernel, self.fused_vars, self.fused_muts, self.lens)
         self.total_elements = sum(self.lens)

@@ -62,6 +62,7 @@ def _lib():
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(i)]
         h.hipSetDevice.argtypes = [i]
         h.hipGetDevice.argtypes = [ctypes.POINTER(i)]
+        h.hipDeviceGetName.argtypes = [ctypes.c_char_p, i, i]
         h.hipGetErrorString.argtypes = [i]
         h.hipGetErrorString.restype = ctypes.c_char_p
         h.hipDeviceSynchronize.argtypes = []
@@ -104,6 +105,12 @@ def get_device() -> int:
     d = ctypes.c_int(0)
     check(_lib().hipGetDevice(ctypes.byref(d)), "hipGetDevice")
     return d.value
+
+
+def device_name(dev: int) -> str:
+    buf = ctypes.create_string_buffer(256)
+    check(_lib().hipDeviceGetName(buf, 256, dev), "hipDeviceGetName")
+    return buf.value.decode(errors="replace")
 
 
 def synchronize() -> None:
