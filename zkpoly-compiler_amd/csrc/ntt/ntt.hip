@@ -286,80 +286,104 @@ __device__ __forceinline__ void lds_put(uint4* l0, uint4* l1, uint32_t p, const 
   l1[p] = make_uint4(x.v[4], x.v[5], x.v[6], x.v[7]);
 }
 
+constexpr uint32_t HEAD_BUF = HEAD_W * HEAD_PITCH;  // one 32 x 33 tile plane
+constexpr int HEAD_LOADS = HEAD_W * HEAD_W / THREADS;
+
+// Two tile buffers per plane (x at M at offset 0, x at rev M at HEAD_BUF) and
+// the head's twiddles r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1).
 struct HeadLds {
-  uint4 A0[HEAD_W * HEAD_PITCH], A1[HEAD_W * HEAD_PITCH];
-  uint4 B0[HEAD_W * HEAD_PITCH], B1[HEAD_W * HEAD_PITCH];
+  uint4 p0[2 * HEAD_BUF], p1[2 * HEAD_BUF];
+  uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
 };
 
-// Twiddle of DIT stage m for j < 2^m: w^(j * 2^(log_n - 1 - m)), read from the
-// caller's table [1, w, ..., w^(n/2 - 1)].
-struct TableTw {
-  const uint32_t* tw;
-  uint32_t log_n;
-  __device__ __forceinline__ Fr get(uint32_t m, uint32_t j) const {
-    return Fr::load(tw + (((uint64_t)j << (log_n - 1 - m)) << 3));
-  }
-};
-
-// The same twiddles from an LDS table r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1),
-// for stages m < t.
-struct LdsTw {
-  const uint4* r0;
-  const uint4* r1;
-  uint32_t t;
-  __device__ __forceinline__ Fr get(uint32_t m, uint32_t j) const { return lds_get(r0, r1, j << (t - 1 - m)); }
-};
-
-template <class TW>
-__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const TW& tws, uint32_t log_n, uint32_t t,
-                                          uint64_t M) {
-  uint4* A0 = L.A0;
-  uint4* A1 = L.A1;
-  uint4* B0 = L.B0;
-  uint4* B1 = L.B1;
+// The caller has written L.r (visible after the first barrier below).
+__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint64_t M) {
   const uint32_t W = 1u << t;
   const uint32_t mbits = log_n - 2 * t;
   const uint64_t Mr = mbits ? (__brevll(M) >> (64 - mbits)) : 0;
   if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
   const bool pair = M != Mr;
   const uint32_t shift_a = log_n - t;
-  for (uint32_t e = threadIdx.x; e < W * W; e += THREADS) {
+  // all global loads of the thread first, then the LDS stores; the loads are
+  // unconditional (index clamped for t < 5, x at M read twice when M == rev M)
+  // and held in named values, so the staging stays in registers
+  struct Staged {
+    uint4 a0, a1, b0, b1;
+  };
+  auto fetch = [&](uint32_t j) {
+    const uint32_t e = (threadIdx.x + j * THREADS) & (W * W - 1);
     const uint32_t a = e >> t, b = e & (W - 1);
-    const uint32_t p = a * HEAD_PITCH + b;
     const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
-    A0[p] = s[0];
-    A1[p] = s[1];
-    if (pair) {
-      const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
-      B0[p] = q[0];
-      B1[p] = q[1];
+    const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
+    return Staged{s[0], s[1], q[0], q[1]};
+  };
+  auto place = [&](uint32_t j, const Staged& g) {
+    const uint32_t e = threadIdx.x + j * THREADS;
+    if (e < W * W) {
+      const uint32_t p = (e >> t) * HEAD_PITCH + (e & (W - 1));
+      L.p0[p] = g.a0;
+      L.p1[p] = g.a1;
+      if (pair) {
+        L.p0[HEAD_BUF + p] = g.b0;
+        L.p1[HEAD_BUF + p] = g.b1;
+      }
     }
-  }
+  };
+  static_assert(HEAD_LOADS == 4, "head staging is written for four loads per thread");
+  const Staged g0 = fetch(0), g1 = fetch(1), g2 = fetch(2), g3 = fetch(3);
+  place(0, g0);
+  place(1, g1);
+  place(2, g2);
+  place(3, g3);
   __syncthreads();
-  // tile 0 = y at M (stored in the buffer of x at Mr), tile 1 = y at Mr (buffer of x at M)
-  uint4* Y0a = pair ? B0 : A0;
-  uint4* Y0b = pair ? B1 : A1;
-  const uint32_t tiles = pair ? 2 : 1;
+  // tile 0 = y at M (in the buffer of x at Mr), tile 1 = y at Mr (buffer of x at M)
+  const uint32_t y0 = pair ? HEAD_BUF : 0;
   const uint32_t per_tile = W * (W >> 1);
+  const uint32_t total = (pair ? 2 : 1) * per_tile;
+  // w^0 is 1 in every table built as powers; the product by it is skipped
+  // only when the table holds exactly 1, so any other table is still honoured
+  const bool unit0 = lds_get(L.r0, L.r1, 0) == Fr::one();
   for (uint32_t m = 0; m < t; m++) {
     const uint32_t half = 1u << m;
-    for (uint32_t u = threadIdx.x; u < tiles * per_tile; u += THREADS) {
-      const uint32_t tile = u >= per_tile ? 1 : 0;
-      const uint32_t r = u - tile * per_tile;
-      const uint32_t a = r >> (t - 1);
-      const uint32_t q = r & ((W >> 1) - 1);
-      const uint32_t blo = ((q >> m) << (m + 1)) | (q & (half - 1));
-      const uint32_t ra = rev_bits(a, t);
-      const uint32_t pa = rev_bits(blo, t) * HEAD_PITCH + ra;
-      const uint32_t pb = rev_bits(blo + half, t) * HEAD_PITCH + ra;
-      uint4* L0 = tile ? A0 : Y0a;
-      uint4* L1 = tile ? A1 : Y0b;
-      const Fr w = tws.get(m, blo & (half - 1));
-      const Fr x = lds_get(L0, L1, pa);
-      const Fr y = lds_get(L0, L1, pb);
-      const Fr tt = y * w;
-      lds_put(L0, L1, pa, x + tt);
-      lds_put(L0, L1, pb, x - tt);
+    const bool skip = m == 0 && unit0;
+    for (uint32_t u0 = threadIdx.x; u0 < total; u0 += 4 * THREADS) {
+      Fr x[4], y[4];
+      uint32_t pa[4], pb[4], wi[4];
+      bool live[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t u = u0 + j * THREADS;
+        live[j] = u < total;
+        const uint32_t uu = live[j] ? u : u0;
+        const uint32_t tile = uu >= per_tile ? 1 : 0;
+        const uint32_t r = uu - tile * per_tile;
+        const uint32_t a = r >> (t - 1);
+        const uint32_t q = r & ((W >> 1) - 1);
+        const uint32_t blo = ((q >> m) << (m + 1)) | (q & (half - 1));
+        const uint32_t off = tile ? 0 : y0;
+        const uint32_t ra = rev_bits(a, t);
+        pa[j] = off + rev_bits(blo, t) * HEAD_PITCH + ra;
+        pb[j] = off + rev_bits(blo + half, t) * HEAD_PITCH + ra;
+        wi[j] = (blo & (half - 1)) << (t - 1 - m);
+        x[j] = lds_get(L.p0, L.p1, pa[j]);
+        y[j] = lds_get(L.p0, L.p1, pb[j]);
+      }
+      if (!skip) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) y[j] = y[j] * lds_get(L.r0, L.r1, wi[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const Fr tt = y[j];
+        y[j] = x[j] - tt;
+        x[j] = x[j] + tt;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (!live[j]) continue;
+        lds_put(L.p0, L.p1, pa[j], x[j]);
+        lds_put(L.p0, L.p1, pb[j], y[j]);
+      }
     }
     __syncthreads();
   }
@@ -367,20 +391,28 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, const T
     const uint32_t a = e >> t, b = e & (W - 1);
     const uint32_t p = rev_bits(b, t) * HEAD_PITCH + rev_bits(a, t);
     uint4* d = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
-    d[0] = Y0a[p];
-    d[1] = Y0b[p];
+    d[0] = L.p0[y0 + p];
+    d[1] = L.p1[y0 + p];
     if (pair) {
       uint4* d2 = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
-      d2[0] = A0[p];
-      d2[1] = A1[p];
+      d2[0] = L.p0[p];
+      d2[1] = L.p1[p];
     }
   }
+}
+
+// L.r[i] = tw[i << (log_n - t)] for i < 2^(t - 1)
+__device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint32_t* __restrict__ tw, uint32_t log_n,
+                                                         uint32_t t) {
+  if (threadIdx.x < ((1u << t) >> 1))
+    lds_put(L.r0, L.r1, threadIdx.x, Fr::load(tw + (((uint64_t)threadIdx.x << (log_n - t)) << 3)));
 }
 
 __global__ void __launch_bounds__(THREADS)
     bitrev_head(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t) {
   __shared__ HeadLds L;
-  head_tile(L, v, TableTw{tw, log_n}, log_n, t, blockIdx.x);
+  head_twiddles_from_table(L, tw, log_n, t);
+  head_tile(L, v, log_n, t, blockIdx.x);
 }
 
 // Fused bit reversal + first head_t stages of every instance; instance k owns
@@ -393,7 +425,9 @@ __global__ void __launch_bounds__(THREADS)
   const uint64_t b = blockIdx.x;
   if (b >= head_prefix[count]) return;
   const uint32_t k = find_instance(head_prefix, count, b);
-  head_tile(L, inst[k].v, TableTw{inst[k].tw, inst[k].log_n}, inst[k].log_n, inst[k].head_t, b - head_prefix[k]);
+  const BatchInstance I = inst[k];
+  head_twiddles_from_table(L, I.tw, I.log_n, I.head_t);
+  head_tile(L, I.v, I.log_n, I.head_t, b - head_prefix[k]);
 }
 
 // ---------------------------------------------------------------- radix-4 pass
@@ -611,12 +645,11 @@ __global__ void __launch_bounds__(THREADS)
     rbitrev_head(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg, const uint32_t* __restrict__ omegas,
                  uint32_t log_n, uint32_t t) {
   __shared__ HeadLds L;
-  __shared__ uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
   const uint64_t M = blockIdx.x;
   const uint32_t mbits = log_n - 2 * t;
   if (mbits && M > (__brevll(M) >> (64 - mbits))) return;  // uniform per block
-  build_small_roots(r0, r1, pq, pq_deg, omegas, log_n, t);
-  head_tile(L, v, LdsTw{r0, r1, t}, log_n, t, M);
+  build_small_roots(L.r0, L.r1, pq, pq_deg, omegas, log_n, t);
+  head_tile(L, v, log_n, t, M);
 }
 
 struct RPassLds {
