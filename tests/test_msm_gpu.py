@@ -169,3 +169,26 @@ def test_workload_2_20_matches_best_multiexp(libs, stream):
     pb = gpu_bases(libs, stream, ks)
     bases = pb.download(np.uint64, n * 8).reshape(n, 8)
     assert list(run_device(libs, stream, pb, ss, 16)) == list(orc.best_multiexp(ss, bases))
+
+
+@pytest.mark.parametrize("window", [2, 5, 8, 20, 21])
+def test_narrow_and_wide_windows(libs, stream, window):
+    """Windows at both ends of the partition sort (2: tiles of 64 points and 128
+    windows; 20: 4096 low bins) and past it (21: the atomic counting sort)."""
+    n = 3000
+    ks = random_elements(n, 6000 + window)
+    ss = random_elements(n, 6100 + window)
+    pb = gpu_bases(libs, stream, ks)
+    assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss))
+
+
+@pytest.mark.parametrize("value", [1, FR_MODULUS - 1, 12345])
+def test_all_scalars_equal(libs, stream, value):
+    """Every scalar the same: each window's entries land in one bucket, one sort
+    segment holds them all, and the result is value * sum P_i."""
+    n = 1 << 15
+    ks = random_elements(n, 7000)
+    ss = np.broadcast_to(mont_limbs(value), (n, 4)).copy()
+    pb = gpu_bases(libs, stream, ks)
+    for window in (16, 13):
+        assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss)), window

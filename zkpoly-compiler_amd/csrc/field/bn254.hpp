@@ -6,10 +6,12 @@
 // hands to its kernels (core/src/ntt.rs:186, core/src/msm.rs:325; limb layout
 // core/src/common/mont/src/field.cuh:637-644).
 //
-// Multiplication is CIOS Montgomery on 32-bit limbs, written so that every
-// limb product is one 32x32+64 -> 64 multiply-add (v_mad_u64_u32 on CDNA).
-// The "no final carry word" form is valid because the top limb of both moduli
-// is below 2^31 - 1, so the running value never exceeds 2p < 2^255.
+// Multiplication (device): product scanning on 32-bit limbs with interleaved
+// Montgomery reduction; every limb product is one v_mad_u64_u32 whose carry-out
+// is folded into a third accumulator word.  Host: CIOS on 64-bit limbs with
+// 128-bit intermediates.  Both return the canonical representative, so they
+// agree bit for bit.  The top limb of both moduli is below 2^30, so the
+// running value never exceeds 2p < 2^255.
 // Semantics follow the reference's mont::Element (field.cuh:907-1204):
 // canonical output, zero-preserving negation, Fermat inversion.
 #pragma once
@@ -178,9 +180,57 @@ struct Fp {
 #if defined(__HIP_DEVICE_COMPILE__)
     return mul_ps(*this, b);
 #else
-    return mul_cios(*this, b);
+    return mul_cios64(*this, b);
 #endif
   }
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+  // -p^-1 mod 2^64 by Newton iteration on the low limb.
+  static constexpr u64 inv64() {
+    const u64 p0 = (u64)Params::P[0] | ((u64)Params::P[1] << 32);
+    u64 x = 1;
+    for (int i = 0; i < 6; i++) x *= 2 - p0 * x;  // x = p0^-1 mod 2^64
+    return (u64)0 - x;
+  }
+  // Host CIOS on 4 x 64-bit limbs (same memory layout as 8 x 32).
+  static Fp mul_cios64(const Fp& a, const Fp& b) {
+    typedef unsigned __int128 u128;
+    u64 A[4], B[4], P[4], t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+      A[i] = (u64)a.v[2 * i] | ((u64)a.v[2 * i + 1] << 32);
+      B[i] = (u64)b.v[2 * i] | ((u64)b.v[2 * i + 1] << 32);
+      P[i] = (u64)Params::P[2 * i] | ((u64)Params::P[2 * i + 1] << 32);
+    }
+    constexpr u64 INV64 = inv64();
+    for (int i = 0; i < 4; i++) {
+      u128 c = 0;
+      for (int j = 0; j < 4; j++) {
+        c += (u128)A[j] * B[i] + t[j];
+        t[j] = (u64)c;
+        c >>= 64;
+      }
+      u128 s = (u128)t[4] + c;
+      t[4] = (u64)s;
+      t[5] = (u64)(s >> 64);
+      const u64 m = t[0] * INV64;
+      c = ((u128)m * P[0] + t[0]) >> 64;
+      for (int j = 1; j < 4; j++) {
+        c += (u128)m * P[j] + t[j];
+        t[j - 1] = (u64)c;
+        c >>= 64;
+      }
+      s = (u128)t[4] + c;
+      t[3] = (u64)s;
+      t[4] = t[5] + (u64)(s >> 64);
+    }
+    Fp r;
+    for (int i = 0; i < 4; i++) {
+      r.v[2 * i] = (u32)t[i];
+      r.v[2 * i + 1] = (u32)(t[i] >> 32);
+    }
+    return reduce_once(r);
+  }
+#endif
 
   ZKP_HD static Fp mul_cios(const Fp& a, const Fp& b) {
     u32 t[8];

@@ -12,13 +12,17 @@
 //     [-2^(s-1), 2^(s-1)) (the reference's signed_digit, msm_impl.cuh:22-33);
 //     logical window w uses point table w / n_windows and physical window
 //     w % n_windows (precomputed tables 2^(j*n_windows*s) P, msm_impl.cuh:358-371).
-//  2. counting sort of the (window, |digit|) keys: device-atomic histogram,
-//     exclusive scan, atomic scatter of (point index | sign << 31).
+//  2. sort of the (window, |digit|) keys with values (point index | sign << 31):
+//     a two-level partition sort in LDS (coarse bins, then the low bucket
+//     bits), no device atomics; windows above 20 bits use a device-atomic
+//     counting sort instead.
 //  3. bucket accumulation, load balanced: every thread owns 32 consecutive
 //     sorted entries and adds each run of equal keys into an XYZZ sum (mixed
-//     additions); runs that cross a chunk edge become (key, partial) pairs
-//     that the next level reduces the same way, so a bucket holding any share
-//     of the points (e.g. the scalar-1 bucket) costs log levels, not a serial walk.
+//     additions); a run of at most 64 entries is summed whole by the thread
+//     holding its first entry, a longer one leaves (key, partial) pairs per
+//     chunk that the next levels reduce the same way, so a bucket holding any
+//     share of the points (e.g. the scalar-1 bucket) costs log levels, not a
+//     serial walk.
 //  4. bucket reduction per window, S_w = sum_b b * B_b, in levels of running
 //     sums over segments of 8 (no serial walk over 2^(s-1) buckets).
 //  5. host: Horner over the window sums (s doublings each) and one inversion
@@ -243,6 +247,142 @@ __global__ void __launch_bounds__(THREADS) k_scan_add(uint32_t* data, uint32_t n
   }
 }
 
+// ---------------------------------------------------------------- partition sort
+//
+// Sorts the (key, value) entries by key without device atomics, in two levels.
+// A key is pw * buckets + |d| - 1; its coarse bin is key >> low_bits (the
+// physical window and the top bits of the bucket).
+//  1. k_part_count: every tile of `tp` points counts its entries per coarse
+//     bin in LDS and writes the counts to matrix[bin * tiles + tile]; an
+//     exclusive scan of the matrix gives every (bin, tile) run its place.
+//  2. k_part_scatter: every tile ranks its entries within their bins in LDS,
+//     stages them in bin order and writes each run contiguously.
+//  3. k_seg_sort: one workgroup per coarse bin sorts its segment by the low
+//     bits with an LDS counting sort.  The order inside a bucket is not kept;
+//     bucket sums do not depend on it.
+
+struct SortPlan {
+  uint32_t low_bits;  // bits of the bucket sorted by the second level (<= 12)
+  uint32_t nc;        // coarse bins: n_windows << (s - 1 - low_bits)
+  uint32_t tp;        // points per tile
+  uint32_t tiles;
+  uint32_t lds_part;  // dynamic LDS of k_part_scatter (bytes)
+};
+
+// Counts the tile's nonzero digits per coarse bin into cnt (LDS, zeroed by the caller).
+__device__ __forceinline__ void part_count_tile(const uint32_t* __restrict__ scalars, uint64_t n, const Config& cfg,
+                                                uint32_t low_bits, uint64_t first, uint32_t tp, uint32_t* cnt) {
+  for (uint32_t p = threadIdx.x; p < tp; p += THREADS) {
+    const uint64_t i = first + p;
+    if (i >= n) break;
+    const Fr c = Fr::load(scalars + i * 8).to_canonical();
+    int carry = 0;
+    for (uint32_t w = 0; w < cfg.actual_windows; w++) {
+      const int d = next_digit(c, w, cfg.s, carry);
+      if (d == 0) continue;
+      const uint32_t key = (w % cfg.n_windows) * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+      atomicAdd(&cnt[key >> low_bits], 1u);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(THREADS)
+    k_part_count(const uint32_t* __restrict__ scalars, uint64_t n, Config cfg, SortPlan sp,
+                 uint32_t* __restrict__ matrix) {
+  extern __shared__ uint32_t dyn[];
+  uint32_t* cnt = dyn;  // nc
+  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) cnt[b] = 0;
+  __syncthreads();
+  part_count_tile(scalars, n, cfg, sp.low_bits, (uint64_t)blockIdx.x * sp.tp, sp.tp, cnt);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) matrix[(uint64_t)b * sp.tiles + blockIdx.x] = cnt[b];
+}
+
+// Exclusive scan of a[0..m) in LDS by the whole block; sh is THREADS words of scratch.
+__device__ __forceinline__ void lds_exclusive_scan(uint32_t* a, uint32_t m, uint32_t* sh) {
+  const uint32_t per = (m + THREADS - 1) / THREADS;
+  const uint32_t base = threadIdx.x * per;
+  uint32_t local = 0;
+  for (uint32_t k = 0; k < per; k++)
+    if (base + k < m) local += a[base + k];
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan(local, sh, &total);
+  for (uint32_t k = 0; k < per; k++) {
+    if (base + k < m) {
+      const uint32_t x = a[base + k];
+      a[base + k] = pre;
+      pre += x;
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(THREADS)
+    k_part_scatter(const uint32_t* __restrict__ scalars, uint64_t n, Config cfg, SortPlan sp,
+                   const uint32_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+                   uint32_t* __restrict__ vals_out) {
+  extern __shared__ uint32_t dyn[];
+  __shared__ uint32_t sh[THREADS];
+  uint32_t* base = dyn;               // nc: first staging slot of each bin
+  uint32_t* cur = dyn + sp.nc;        // nc: staging cursor
+  uint32_t* skey = dyn + 2 * sp.nc;   // tp * actual_windows
+  uint32_t* sval = skey + (size_t)sp.tp * cfg.actual_windows;
+  const uint64_t first = (uint64_t)blockIdx.x * sp.tp;
+  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) base[b] = 0;
+  __syncthreads();
+  part_count_tile(scalars, n, cfg, sp.low_bits, first, sp.tp, base);
+  __syncthreads();
+  lds_exclusive_scan(base, sp.nc, sh);
+  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) cur[b] = base[b];
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < sp.tp; p += THREADS) {
+    const uint64_t i = first + p;
+    if (i >= n) break;
+    const Fr c = Fr::load(scalars + i * 8).to_canonical();
+    int carry = 0;
+    for (uint32_t w = 0; w < cfg.actual_windows; w++) {
+      const int d = next_digit(c, w, cfg.s, carry);
+      if (d == 0) continue;
+      const uint32_t key = (w % cfg.n_windows) * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+      const uint32_t slot = atomicAdd(&cur[key >> sp.low_bits], 1u);
+      skey[slot] = key;
+      sval[slot] = (uint32_t)((w / cfg.n_windows) * n + i) | (d < 0 ? 0x80000000u : 0u);
+    }
+  }
+  __syncthreads();
+  const uint32_t staged = cur[sp.nc - 1];  // the last bin's cursor ends at the tile's entry count
+  for (uint32_t slot = threadIdx.x; slot < staged; slot += THREADS) {
+    const uint32_t key = skey[slot];
+    const uint32_t b = key >> sp.low_bits;
+    const uint32_t dst = offsets[(uint64_t)b * sp.tiles + blockIdx.x] + (slot - base[b]);
+    keys_out[dst] = key;
+    vals_out[dst] = sval[slot];
+  }
+}
+
+// One workgroup per coarse bin: counting sort of its segment by the low bits.
+__global__ void __launch_bounds__(THREADS)
+    k_seg_sort(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+               const uint32_t* __restrict__ offsets, SortPlan sp, uint32_t* __restrict__ keys_out,
+               uint32_t* __restrict__ vals_out) {
+  extern __shared__ uint32_t h[];  // 2^low_bits
+  __shared__ uint32_t sh[THREADS];
+  const uint32_t nb = 1u << sp.low_bits, mask = nb - 1;
+  const uint32_t start = offsets[(uint64_t)blockIdx.x * sp.tiles];
+  const uint32_t end = offsets[(uint64_t)(blockIdx.x + 1) * sp.tiles];
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) h[b] = 0;
+  __syncthreads();
+  for (uint32_t e = start + threadIdx.x; e < end; e += THREADS) atomicAdd(&h[keys_in[e] & mask], 1u);
+  __syncthreads();
+  lds_exclusive_scan(h, nb, sh);
+  for (uint32_t e = start + threadIdx.x; e < end; e += THREADS) {
+    const uint32_t key = keys_in[e];
+    const uint32_t pos = start + atomicAdd(&h[key & mask], 1u);
+    keys_out[pos] = key;
+    vals_out[pos] = vals_in[e];
+  }
+}
+
 // ---------------------------------------------------------------- accumulation
 
 __device__ __forceinline__ void bucket_add(u32* buckets, uint32_t key, const G1XYZZ& sum) {
@@ -258,15 +398,14 @@ __device__ __forceinline__ G1Affine fetch_point(const u32* __restrict__ points, 
   return a;
 }
 
-// One accumulation level.  Thread t owns sorted entries [t*CHUNK, ...) and
-// sums each run of equal keys.  A run closed on both sides goes to its bucket;
+// One accumulation level over (key, XYZZ partial) entries.  Thread t owns
+// sorted entries [t*CHUNK, ...) and sums each run of equal keys.  A run closed on both sides goes to its bucket;
 // the chunk's first run (if it continues from the previous chunk) and last run
 // (if it continues into the next) are emitted as (key, partial) at slots 2t and
 // 2t + 1 of the next level, identity where there is nothing to carry, so every
 // key's pieces stay contiguous.
-template <bool FROM_POINTS>
-__device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                 const u32* __restrict__ src, uint32_t* counts, uint32_t level,
+__device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ keys, const u32* __restrict__ src,
+                                                 uint32_t* counts, uint32_t level,
                                                  u32* buckets, uint32_t* __restrict__ next_keys,
                                                  u32* __restrict__ next_pts) {
   const uint32_t N = counts[level];
@@ -284,10 +423,7 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
   uint32_t run_start = start;
   for (uint32_t i = start; i < end; i++) {
     const uint32_t key = keys[i];
-    if (FROM_POINTS)
-      acc = acc.add_affine(fetch_point(src, vals[i]));
-    else
-      acc = acc.add(G1XYZZ::load(src + (size_t)i * 32));
+    acc = acc.add(G1XYZZ::load(src + (size_t)i * 32));
     const bool run_end = (i + 1 == end) || keys[i + 1] != key;
     if (run_end) {
       const bool first = run_start == start, last = i + 1 == end;
@@ -312,19 +448,86 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
   }
 }
 
-// Level 0: entries are (key, point index | sign << 31).
+// Runs (buckets) of at most RUN_MAX sorted entries are summed whole by one
+// thread at level 0; only longer runs leave partials for the next levels.
+constexpr uint32_t RUN_MAX = 2 * CHUNK;
+
+// Whether the run of `key` through position pos has at most RUN_MAX entries.
+// Every position of a run gives the same answer.
+__device__ __forceinline__ bool run_is_short(const uint32_t* __restrict__ keys, uint32_t N, uint32_t pos,
+                                             uint32_t key) {
+  uint32_t len = 1;
+  for (uint32_t j = pos; j > 0 && keys[j - 1] == key; j--)
+    if (++len > RUN_MAX) return false;
+  for (uint32_t j = pos + 1; j < N && keys[j] == key; j++)
+    if (++len > RUN_MAX) return false;
+  return true;
+}
+
+// Level 0: entries are (key, point index | sign << 31).  Thread t owns sorted
+// entries [t*CHUNK, ...).  A short run is summed by the thread whose chunk holds
+// its first entry, past the chunk end if need be, and added to its bucket; the
+// other threads skip it.  A long run leaves one partial per chunk it touches
+// (slot 2t for the piece continuing from the previous chunk, 2t + 1 for the
+// piece continuing into the next), as accumulate_chunk does at later levels.
 __global__ void __launch_bounds__(THREADS)
     k_accumulate0(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                   const u32* __restrict__ points, uint32_t* counts, u32* buckets, uint32_t* __restrict__ next_keys,
                   u32* __restrict__ next_pts) {
-  accumulate_chunk<true>(keys, vals, points, counts, 0, buckets, next_keys, next_pts);
+  const uint32_t N = counts[0];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nchunks = (N + CHUNK - 1) / CHUNK;
+  if (t == 0) counts[1] = 2 * nchunks;
+  if (t >= nchunks) return;
+  const uint32_t start = t * CHUNK;
+  const uint32_t end = start + CHUNK < N ? start + CHUNK : N;
+  const uint32_t key_first = keys[start], key_last = keys[end - 1];
+  const bool open_left = start > 0 && keys[start - 1] == key_first;
+  const bool open_right = end < N && keys[end] == key_last;
+  bool a_done = false, b_done = false;
+  uint32_t i = start;
+  if (open_left) {
+    uint32_t r1 = start;
+    while (r1 < end && keys[r1] == key_first) r1++;
+    if (!run_is_short(keys, N, start, key_first)) {
+      G1XYZZ acc = G1XYZZ::identity();
+      for (uint32_t j = start; j < r1; j++) acc = acc.add_affine(fetch_point(points, vals[j]));
+      if (next_pts) acc.store(next_pts + (size_t)(2 * t) * 32);
+      a_done = true;
+    }
+    i = r1;  // a short left run belongs to an earlier thread
+  }
+  G1XYZZ acc = G1XYZZ::identity();
+  for (; i < end; i++) {
+    const uint32_t key = keys[i];
+    acc = acc.add_affine(fetch_point(points, vals[i]));
+    if (i + 1 < end && keys[i + 1] == key) continue;
+    if (i + 1 == end && open_right) {  // the run continues into the next chunk
+      if (run_is_short(keys, N, i, key)) {
+        for (uint32_t j = end; j < N && keys[j] == key; j++) acc = acc.add_affine(fetch_point(points, vals[j]));
+        bucket_add(buckets, key, acc);
+      } else {
+        if (next_pts) acc.store(next_pts + (size_t)(2 * t + 1) * 32);
+        b_done = true;
+      }
+    } else {
+      bucket_add(buckets, key, acc);
+    }
+    acc = G1XYZZ::identity();
+  }
+  if (next_keys) {
+    next_keys[2 * t] = key_first;
+    next_keys[2 * t + 1] = key_last;
+    if (!a_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t) * 32);
+    if (!b_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t + 1) * 32);
+  }
 }
 
 // Level >= 1: entries are (key, XYZZ partial).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate(const uint32_t* __restrict__ keys, const u32* __restrict__ pts, uint32_t* counts, uint32_t level,
                  u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
-  accumulate_chunk<false>(keys, nullptr, pts, counts, level, buckets, next_keys, next_pts);
+  accumulate_chunk(keys, pts, counts, level, buckets, next_keys, next_pts);
 }
 
 __global__ void __launch_bounds__(THREADS) k_zero_u32(uint32_t* p, uint64_t n) {
@@ -410,6 +613,11 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // Device workspace for one MSM of `len` points (per table) and config cfg.
 struct Layout {
   size_t hist, block_sums, cursor, keys0, vals0, counts, lvl_keys[2], lvl_pts[2], buckets, red[4], total;
+  size_t matrix, keys_tmp, vals_tmp;  // partition sort (when `part`)
+  bool part;
+  SortPlan sp;
+  uint64_t matrix_n;  // nc * tiles + 1
+  uint32_t nb_matrix;
   uint32_t H, nb_scan;
   uint64_t n_entries_max;
   uint32_t level_max[MAX_LEVELS];
@@ -418,11 +626,41 @@ struct Layout {
   int red_levels;
 };
 
+// Partition-sort parameters for `len` points; false when the window is too wide
+// for the LDS tables (then the atomic counting sort runs instead).
+constexpr uint32_t PART_LDS_MAX = 96 * 1024;
+static bool make_sort_plan(const Config& cfg, uint64_t len, SortPlan* sp) {
+  const uint32_t bits = cfg.s - 1;  // bucket bits
+  uint32_t cb = bits < 5 ? bits : 5;
+  uint32_t low = bits - cb;
+  if (low > 12) {
+    low = 12;
+    cb = bits - 12;
+  }
+  if (cb > 7) return false;  // windows above 20 bits
+  sp->low_bits = low;
+  sp->nc = cfg.n_windows << cb;
+  uint32_t tp = 1024;
+  while (tp > 64 && (2ull * sp->nc + 2ull * tp * cfg.actual_windows) * 4 > PART_LDS_MAX) tp >>= 1;
+  if ((2ull * sp->nc + 2ull * tp * cfg.actual_windows) * 4 > PART_LDS_MAX) return false;
+  sp->tp = tp;
+  sp->tiles = (uint32_t)((len + tp - 1) / tp);
+  sp->lds_part = (uint32_t)((2ull * sp->nc + 2ull * tp * cfg.actual_windows) * 4);
+  return sp->tiles > 0;
+}
+
 static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   memset(L, 0, sizeof(*L));
   L->H = cfg.n_windows * cfg.buckets;
   L->nb_scan = div_ceil32((uint64_t)L->H + 1, SCAN_BLOCK);
-  if (L->nb_scan > THREADS * 64) return false;
+  if (L->nb_scan > THREADS * 1024) return false;
+  L->part = len > 0 && make_sort_plan(cfg, len, &L->sp);
+  if (L->part) {
+    L->matrix_n = (uint64_t)L->sp.nc * L->sp.tiles + 1;
+    if (L->matrix_n >= 0xffffffffull) return false;
+    L->nb_matrix = div_ceil32(L->matrix_n, SCAN_BLOCK);
+    if (L->nb_matrix > THREADS * 1024) L->part = false;
+  }
   L->n_entries_max = len * cfg.actual_windows;
   if (len * cfg.n_precompute >= 0x80000000ull || L->n_entries_max >= 0xffffffffull) return false;
   // accumulation level sizes (upper bounds)
@@ -455,7 +693,12 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   size_t off = 0;
   L->hist = off; off = align256(off + ((size_t)L->H + 1) * 4);
   L->cursor = off; off = align256(off + ((size_t)L->H + 1) * 4);
-  L->block_sums = off; off = align256(off + (size_t)L->nb_scan * 4);
+  L->block_sums = off; off = align256(off + (size_t)(L->nb_scan > L->nb_matrix ? L->nb_scan : L->nb_matrix) * 4);
+  if (L->part) {
+    L->matrix = off; off = align256(off + (size_t)L->matrix_n * 4);
+    L->keys_tmp = off; off = align256(off + L->n_entries_max * 4);
+    L->vals_tmp = off; off = align256(off + L->n_entries_max * 4);
+  }
   L->counts = off; off = align256(off + MAX_LEVELS * 4 * 2);
   L->keys0 = off; off = align256(off + L->n_entries_max * 4);
   L->vals0 = off; off = align256(off + L->n_entries_max * 4);
@@ -524,6 +767,39 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
   uint32_t* keys0 = reinterpret_cast<uint32_t*>(ws + L.keys0);
   uint32_t* vals0 = reinterpret_cast<uint32_t*>(ws + L.vals0);
   u32* buckets = reinterpret_cast<u32*>(ws + L.buckets);
+  if (L.part) {
+    // partition sort (no device atomics); `len` may be below the planned maximum
+    SortPlan sp = L.sp;
+    sp.tiles = (uint32_t)((len + sp.tp - 1) / sp.tp);
+    const uint64_t mn = (uint64_t)sp.nc * sp.tiles + 1;
+    uint32_t* matrix = reinterpret_cast<uint32_t*>(ws + L.matrix);
+    uint32_t* ktmp = reinterpret_cast<uint32_t*>(ws + L.keys_tmp);
+    uint32_t* vtmp = reinterpret_cast<uint32_t*>(ws + L.vals_tmp);
+    pt.mark("start");
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(THREADS), 0, stream, matrix + (mn - 1), (uint64_t)1);
+    hipLaunchKernelGGL(k_part_count, dim3(sp.tiles), dim3(THREADS), sp.nc * 4, stream, d_scalars, len, cfg, sp, matrix);
+    ZKP_CHECK(hipGetLastError());
+    pt.mark("count");
+    const uint32_t nbm = div_ceil32(mn, SCAN_BLOCK);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nbm), dim3(THREADS), 0, stream, matrix, (uint32_t)mn, bsums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(THREADS), 0, stream, bsums, nbm);
+    hipLaunchKernelGGL(k_scan_add, dim3(div_ceil32(mn, THREADS)), dim3(THREADS), 0, stream, matrix, (uint32_t)mn, bsums,
+                       nullptr);
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, stream, counts, matrix, (uint32_t)(mn - 1));
+    ZKP_CHECK(hipGetLastError());
+    pt.mark("scan");
+    if (sp.lds_part > 64 * 1024)
+      ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sp.lds_part));
+    hipLaunchKernelGGL(k_part_scatter, dim3(sp.tiles), dim3(THREADS), sp.lds_part, stream, d_scalars, len, cfg, sp,
+                       matrix, ktmp, vtmp);
+    ZKP_CHECK(hipGetLastError());
+    pt.mark("partition");
+    hipLaunchKernelGGL(k_seg_sort, dim3(sp.nc), dim3(THREADS), (1u << sp.low_bits) * 4, stream, ktmp, vtmp, matrix, sp,
+                       keys0, vals0);
+    ZKP_CHECK(hipGetLastError());
+    pt.mark("segsort");
+  } else {
   const uint32_t h1 = L.H + 1;
   hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(h1, 4096)), dim3(THREADS), 0, stream, hist, (uint64_t)h1);
   ZKP_CHECK(hipGetLastError());
@@ -541,6 +817,7 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
                      keys0, vals0);
   ZKP_CHECK(hipGetLastError());
   pt.mark("scatter");
+  }
   const uint64_t nchunks0 = (L.n_entries_max + CHUNK - 1) / CHUNK;
   // with a single chunk every run is closed: no next level
   uint32_t* nk = L.levels > 0 ? reinterpret_cast<uint32_t*>(ws + L.lvl_keys[0]) : nullptr;
