@@ -189,3 +189,27 @@ def test_msm_config_query():
     assert list(out) == [16, 16, 16, 1, 1 << 15]
     assert lib.zkp_msm_config(1, 16, out) == 0
     assert lib.zkp_msm_workspace_size(1 << 20, 16, 16) > 0
+
+
+def test_msm_config_extra_carry_window_keeps_reference_tables():
+    """For s = 2, 3 the top signed digit of r - 1 carries, so one window more
+    than ceil(254 / s) is needed.  With a target covering the reference's
+    windows it joins them (one table, as the reference's MsmConfig says);
+    where it would need a table the reference does not count, the config is
+    rejected instead of reading past the caller's tables."""
+    from zkpoly_amd.msm import core_lib
+
+    lib = core_lib(Libs())
+    out = (ctypes.c_uint * 5)()
+    assert lib.zkp_msm_config(2, 127, out) == 1
+    assert list(out) == [2, 128, 128, 1, 2]
+    assert lib.zkp_msm_config(3, 85, out) == 1
+    assert list(out) == [3, 86, 86, 1, 4]
+    assert lib.zkp_msm_config(2, 64, out) == 1
+    assert list(out)[2:4] == [64, 2]  # ceil(127 / 64) = 2 tables in the reference too
+    for s, target in ((2, 1), (3, 1), (3, 5), (3, 17)):
+        assert lib.zkp_msm_config(s, target, out) == 0, (s, target)
+    # the windows in use are unaffected
+    for s in range(4, 25):
+        assert lib.zkp_msm_config(s, -(-254 // s), out) == 1
+        assert out[1] == -(-254 // s) and out[3] == 1

@@ -18,11 +18,9 @@
 //     counting sort instead.
 //  3. bucket accumulation, load balanced: every thread owns 32 consecutive
 //     sorted entries and adds each run of equal keys into an XYZZ sum (mixed
-//     additions); a run of at most 64 entries is summed whole by the thread
-//     holding its first entry, a longer one leaves (key, partial) pairs per
-//     chunk that the next levels reduce the same way, so a bucket holding any
-//     share of the points (e.g. the scalar-1 bucket) costs log levels, not a
-//     serial walk.
+//     additions); runs that cross a chunk edge become (key, partial) pairs
+//     that the next level reduces the same way, so a bucket holding any share
+//     of the points (e.g. the scalar-1 bucket) costs log levels, not a serial walk.
 //  4. bucket reduction per window, S_w = sum_b b * B_b, in levels of running
 //     sums over segments of 8 (no serial walk over 2^(s-1) buckets).
 //  5. host: Horner over the window sums (s doublings each) and one inversion
@@ -79,11 +77,19 @@ static uint32_t top_window_of_r_minus_1(uint32_t bit, uint32_t s) {
 static bool make_config(uint32_t s, uint32_t target, Config* c) {
   if (s < 2 || s > 24 || target == 0) return false;
   c->s = s;
-  c->actual_windows = div_ceil32(SCALAR_BITS, s);
-  // the signed recoding must not carry out of the top window: top digit + 1 < 2^(s-1)
+  const uint32_t ref_windows = div_ceil32(SCALAR_BITS, s);  // the reference's count
+  const uint32_t ref_n_windows = ref_windows < target ? ref_windows : target;
+  const uint32_t ref_precompute = div_ceil32(ref_windows, ref_n_windows);
+  c->actual_windows = ref_windows;
+  // the signed recoding must not carry out of the top window: top digit + 1 <
+  // 2^(s-1); where it can (s = 2, 3) one more window is needed
   if (top_window_of_r_minus_1((c->actual_windows - 1) * s, s) + 1 >= (1u << (s - 1))) c->actual_windows += 1;
   c->n_windows = c->actual_windows < target ? c->actual_windows : target;
+  // a target covering the reference's windows covers the extra one too, so the
+  // caller's table count (the reference's MsmConfig) stays right
+  if (target >= ref_windows) c->n_windows = c->actual_windows;
   c->n_precompute = div_ceil32(c->actual_windows, c->n_windows);
+  if (c->n_precompute != ref_precompute) return false;  // would need a table the caller does not have
   c->buckets = 1u << (s - 1);
   return true;
 }
@@ -257,9 +263,10 @@ __global__ void __launch_bounds__(THREADS) k_scan_add(uint32_t* data, uint32_t n
 //     exclusive scan of the matrix gives every (bin, tile) run its place.
 //  2. k_part_scatter: every tile ranks its entries within their bins in LDS,
 //     stages them in bin order and writes each run contiguously.
-//  3. k_seg_sort: one workgroup per coarse bin sorts its segment by the low
-//     bits with an LDS counting sort.  The order inside a bucket is not kept;
-//     bucket sums do not depend on it.
+//  3. k_seg_*: every coarse bin's segment is sorted by the low bits in slices
+//     (LDS counting, one device atomic per nonzero bucket per slice of a
+//     multi-slice segment).  The order inside a bucket is not kept; bucket
+//     sums do not depend on it.
 
 struct SortPlan {
   uint32_t low_bits;  // bits of the bucket sorted by the second level (<= 12)
@@ -360,24 +367,131 @@ __global__ void __launch_bounds__(THREADS)
   }
 }
 
-// One workgroup per coarse bin: counting sort of its segment by the low bits.
+// Second level: every coarse bin's segment is sorted by the low bits in slices
+// of at most SEG_SLICE entries, so a segment holding most entries of a narrow
+// window (s = 12, 13, 14, 18 and the small windows) is spread over many
+// workgroups.  Bucket counts of a one-slice segment are stored, those of a
+// longer one added up with one device atomic per nonzero bucket per slice.
+constexpr uint32_t SEG_SLICE = 16384;
+
+// largest k < n with prefix[k] <= g (prefix: n + 1 non-decreasing entries)
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* prefix, uint32_t n, uint32_t g) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t seg_begin(const uint32_t* offsets, const SortPlan& sp, uint32_t cb) {
+  return offsets[(uint64_t)cb * sp.tiles];
+}
+
+// slice_prefix[cb] = slices of the segments before cb (nc + 1 entries); one workgroup.
 __global__ void __launch_bounds__(THREADS)
-    k_seg_sort(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-               const uint32_t* __restrict__ offsets, SortPlan sp, uint32_t* __restrict__ keys_out,
-               uint32_t* __restrict__ vals_out) {
-  extern __shared__ uint32_t h[];  // 2^low_bits
+    k_seg_plan(const uint32_t* __restrict__ offsets, SortPlan sp, uint32_t* __restrict__ slice_prefix) {
   __shared__ uint32_t sh[THREADS];
+  extern __shared__ uint32_t cnt[];  // nc + 1
+  for (uint32_t cb = threadIdx.x; cb <= sp.nc; cb += THREADS) {
+    uint32_t k = 0;
+    if (cb < sp.nc) {
+      const uint32_t size = seg_begin(offsets, sp, cb + 1) - seg_begin(offsets, sp, cb);
+      k = (size + SEG_SLICE - 1) / SEG_SLICE;
+    }
+    cnt[cb] = k;
+  }
+  __syncthreads();
+  lds_exclusive_scan(cnt, sp.nc + 1, sh);
+  for (uint32_t cb = threadIdx.x; cb <= sp.nc; cb += THREADS) slice_prefix[cb] = cnt[cb];
+}
+
+struct SegSlice {
+  uint32_t cb, lo, hi;  // segment and entry range [lo, hi)
+  bool single;          // the segment is one slice
+};
+
+__device__ __forceinline__ bool seg_slice(const uint32_t* offsets, const uint32_t* slice_prefix, const SortPlan& sp,
+                                          SegSlice* out) {
+  const uint32_t b = blockIdx.x;
+  if (b >= slice_prefix[sp.nc]) return false;
+  const uint32_t cb = find_segment(slice_prefix, sp.nc, b);
+  const uint32_t start = seg_begin(offsets, sp, cb), end = seg_begin(offsets, sp, cb + 1);
+  const uint32_t j = b - slice_prefix[cb];
+  out->cb = cb;
+  out->lo = start + j * SEG_SLICE;
+  out->hi = out->lo + SEG_SLICE < end ? out->lo + SEG_SLICE : end;
+  out->single = slice_prefix[cb + 1] - slice_prefix[cb] == 1;
+  return true;
+}
+
+// ghist[cb * nb + bin] (zeroed beforehand) = entries of the segment in each bucket.
+__global__ void __launch_bounds__(THREADS)
+    k_seg_hist(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ offsets,
+               const uint32_t* __restrict__ slice_prefix, SortPlan sp, uint32_t* __restrict__ ghist) {
+  extern __shared__ uint32_t h[];  // nb
+  SegSlice sl;
+  if (!seg_slice(offsets, slice_prefix, sp, &sl)) return;  // uniform per block
   const uint32_t nb = 1u << sp.low_bits, mask = nb - 1;
-  const uint32_t start = offsets[(uint64_t)blockIdx.x * sp.tiles];
-  const uint32_t end = offsets[(uint64_t)(blockIdx.x + 1) * sp.tiles];
   for (uint32_t b = threadIdx.x; b < nb; b += THREADS) h[b] = 0;
   __syncthreads();
-  for (uint32_t e = start + threadIdx.x; e < end; e += THREADS) atomicAdd(&h[keys_in[e] & mask], 1u);
+  for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) atomicAdd(&h[keys_in[e] & mask], 1u);
+  __syncthreads();
+  uint32_t* g = ghist + (uint64_t)sl.cb * nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) {
+    const uint32_t c = h[b];
+    if (sl.single)
+      g[b] = c;
+    else if (c)
+      atomicAdd(&g[b], c);
+  }
+}
+
+// Per segment: ghist becomes the scatter cursor, the first entry of each bucket.
+__global__ void __launch_bounds__(THREADS)
+    k_seg_scan(const uint32_t* __restrict__ offsets, SortPlan sp, uint32_t* __restrict__ ghist) {
+  extern __shared__ uint32_t h[];  // nb
+  __shared__ uint32_t sh[THREADS];
+  const uint32_t nb = 1u << sp.low_bits;
+  const uint32_t cb = blockIdx.x;
+  const uint32_t start = seg_begin(offsets, sp, cb);
+  uint32_t* g = ghist + (uint64_t)cb * nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) h[b] = g[b];
   __syncthreads();
   lds_exclusive_scan(h, nb, sh);
-  for (uint32_t e = start + threadIdx.x; e < end; e += THREADS) {
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) g[b] = start + h[b];
+}
+
+// Every slice takes its run of each bucket from the cursor (one atomic per
+// nonzero bucket; a one-slice segment owns the whole bucket) and ranks its
+// entries in LDS.  The order inside a bucket is not kept.
+__global__ void __launch_bounds__(THREADS)
+    k_seg_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ slice_prefix, SortPlan sp,
+                  uint32_t* __restrict__ cursor, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+  extern __shared__ uint32_t h[];  // 2 * nb: counts, then bases
+  SegSlice sl;
+  if (!seg_slice(offsets, slice_prefix, sp, &sl)) return;
+  const uint32_t nb = 1u << sp.low_bits, mask = nb - 1;
+  uint32_t* cnt = h;
+  uint32_t* base = h + nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = 0;
+  __syncthreads();
+  for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) atomicAdd(&cnt[keys_in[e] & mask], 1u);
+  __syncthreads();
+  uint32_t* cur = cursor + (uint64_t)sl.cb * nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += THREADS) {
+    const uint32_t c = cnt[b];
+    base[b] = sl.single ? cur[b] : (c ? atomicAdd(&cur[b], c) : 0u);
+    cnt[b] = 0;
+  }
+  __syncthreads();
+  for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) {
     const uint32_t key = keys_in[e];
-    const uint32_t pos = start + atomicAdd(&h[key & mask], 1u);
+    const uint32_t pos = base[key & mask] + atomicAdd(&cnt[key & mask], 1u);
     keys_out[pos] = key;
     vals_out[pos] = vals_in[e];
   }
@@ -398,14 +512,16 @@ __device__ __forceinline__ G1Affine fetch_point(const u32* __restrict__ points, 
   return a;
 }
 
-// One accumulation level over (key, XYZZ partial) entries.  Thread t owns
-// sorted entries [t*CHUNK, ...) and sums each run of equal keys.  A run closed on both sides goes to its bucket;
+// One accumulation level.  Thread t owns sorted entries [t*CHUNK, ...) and
+// sums each run of equal keys, one entry per loop step in every lane (a wave
+// takes CHUNK steps whatever the run lengths).  A run closed on both sides goes to its bucket;
 // the chunk's first run (if it continues from the previous chunk) and last run
 // (if it continues into the next) are emitted as (key, partial) at slots 2t and
 // 2t + 1 of the next level, identity where there is nothing to carry, so every
 // key's pieces stay contiguous.
-__device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ keys, const u32* __restrict__ src,
-                                                 uint32_t* counts, uint32_t level,
+template <bool FROM_POINTS>
+__device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                 const u32* __restrict__ src, uint32_t* counts, uint32_t level,
                                                  u32* buckets, uint32_t* __restrict__ next_keys,
                                                  u32* __restrict__ next_pts) {
   const uint32_t N = counts[level];
@@ -423,7 +539,10 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
   uint32_t run_start = start;
   for (uint32_t i = start; i < end; i++) {
     const uint32_t key = keys[i];
-    acc = acc.add(G1XYZZ::load(src + (size_t)i * 32));
+    if (FROM_POINTS)
+      acc = acc.add_affine(fetch_point(src, vals[i]));
+    else
+      acc = acc.add(G1XYZZ::load(src + (size_t)i * 32));
     const bool run_end = (i + 1 == end) || keys[i + 1] != key;
     if (run_end) {
       const bool first = run_start == start, last = i + 1 == end;
@@ -448,86 +567,19 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
   }
 }
 
-// Runs (buckets) of at most RUN_MAX sorted entries are summed whole by one
-// thread at level 0; only longer runs leave partials for the next levels.
-constexpr uint32_t RUN_MAX = 2 * CHUNK;
-
-// Whether the run of `key` through position pos has at most RUN_MAX entries.
-// Every position of a run gives the same answer.
-__device__ __forceinline__ bool run_is_short(const uint32_t* __restrict__ keys, uint32_t N, uint32_t pos,
-                                             uint32_t key) {
-  uint32_t len = 1;
-  for (uint32_t j = pos; j > 0 && keys[j - 1] == key; j--)
-    if (++len > RUN_MAX) return false;
-  for (uint32_t j = pos + 1; j < N && keys[j] == key; j++)
-    if (++len > RUN_MAX) return false;
-  return true;
-}
-
-// Level 0: entries are (key, point index | sign << 31).  Thread t owns sorted
-// entries [t*CHUNK, ...).  A short run is summed by the thread whose chunk holds
-// its first entry, past the chunk end if need be, and added to its bucket; the
-// other threads skip it.  A long run leaves one partial per chunk it touches
-// (slot 2t for the piece continuing from the previous chunk, 2t + 1 for the
-// piece continuing into the next), as accumulate_chunk does at later levels.
+// Level 0: entries are (key, point index | sign << 31).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate0(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                   const u32* __restrict__ points, uint32_t* counts, u32* buckets, uint32_t* __restrict__ next_keys,
                   u32* __restrict__ next_pts) {
-  const uint32_t N = counts[0];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nchunks = (N + CHUNK - 1) / CHUNK;
-  if (t == 0) counts[1] = 2 * nchunks;
-  if (t >= nchunks) return;
-  const uint32_t start = t * CHUNK;
-  const uint32_t end = start + CHUNK < N ? start + CHUNK : N;
-  const uint32_t key_first = keys[start], key_last = keys[end - 1];
-  const bool open_left = start > 0 && keys[start - 1] == key_first;
-  const bool open_right = end < N && keys[end] == key_last;
-  bool a_done = false, b_done = false;
-  uint32_t i = start;
-  if (open_left) {
-    uint32_t r1 = start;
-    while (r1 < end && keys[r1] == key_first) r1++;
-    if (!run_is_short(keys, N, start, key_first)) {
-      G1XYZZ acc = G1XYZZ::identity();
-      for (uint32_t j = start; j < r1; j++) acc = acc.add_affine(fetch_point(points, vals[j]));
-      if (next_pts) acc.store(next_pts + (size_t)(2 * t) * 32);
-      a_done = true;
-    }
-    i = r1;  // a short left run belongs to an earlier thread
-  }
-  G1XYZZ acc = G1XYZZ::identity();
-  for (; i < end; i++) {
-    const uint32_t key = keys[i];
-    acc = acc.add_affine(fetch_point(points, vals[i]));
-    if (i + 1 < end && keys[i + 1] == key) continue;
-    if (i + 1 == end && open_right) {  // the run continues into the next chunk
-      if (run_is_short(keys, N, i, key)) {
-        for (uint32_t j = end; j < N && keys[j] == key; j++) acc = acc.add_affine(fetch_point(points, vals[j]));
-        bucket_add(buckets, key, acc);
-      } else {
-        if (next_pts) acc.store(next_pts + (size_t)(2 * t + 1) * 32);
-        b_done = true;
-      }
-    } else {
-      bucket_add(buckets, key, acc);
-    }
-    acc = G1XYZZ::identity();
-  }
-  if (next_keys) {
-    next_keys[2 * t] = key_first;
-    next_keys[2 * t + 1] = key_last;
-    if (!a_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t) * 32);
-    if (!b_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t + 1) * 32);
-  }
+  accumulate_chunk<true>(keys, vals, points, counts, 0, buckets, next_keys, next_pts);
 }
 
 // Level >= 1: entries are (key, XYZZ partial).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate(const uint32_t* __restrict__ keys, const u32* __restrict__ pts, uint32_t* counts, uint32_t level,
                  u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
-  accumulate_chunk(keys, pts, counts, level, buckets, next_keys, next_pts);
+  accumulate_chunk<false>(keys, nullptr, pts, counts, level, buckets, next_keys, next_pts);
 }
 
 __global__ void __launch_bounds__(THREADS) k_zero_u32(uint32_t* p, uint64_t n) {
@@ -613,7 +665,8 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // Device workspace for one MSM of `len` points (per table) and config cfg.
 struct Layout {
   size_t hist, block_sums, cursor, keys0, vals0, counts, lvl_keys[2], lvl_pts[2], buckets, red[4], total;
-  size_t matrix, keys_tmp, vals_tmp;  // partition sort (when `part`)
+  size_t matrix, keys_tmp, vals_tmp, slices;  // partition sort (when `part`)
+  uint32_t max_slices;
   bool part;
   SortPlan sp;
   uint64_t matrix_n;  // nc * tiles + 1
@@ -698,6 +751,8 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
     L->matrix = off; off = align256(off + (size_t)L->matrix_n * 4);
     L->keys_tmp = off; off = align256(off + L->n_entries_max * 4);
     L->vals_tmp = off; off = align256(off + L->n_entries_max * 4);
+    L->slices = off; off = align256(off + ((size_t)L->sp.nc + 1) * 4);
+    L->max_slices = (uint32_t)((L->n_entries_max + SEG_SLICE - 1) / SEG_SLICE) + L->sp.nc;
   }
   L->counts = off; off = align256(off + MAX_LEVELS * 4 * 2);
   L->keys0 = off; off = align256(off + L->n_entries_max * 4);
@@ -795,8 +850,14 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
                        matrix, ktmp, vtmp);
     ZKP_CHECK(hipGetLastError());
     pt.mark("partition");
-    hipLaunchKernelGGL(k_seg_sort, dim3(sp.nc), dim3(THREADS), (1u << sp.low_bits) * 4, stream, ktmp, vtmp, matrix, sp,
-                       keys0, vals0);
+    const uint32_t nbk = 1u << sp.low_bits;
+    uint32_t* slices = reinterpret_cast<uint32_t*>(ws + L.slices);
+    hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(THREADS), (sp.nc + 1) * 4, stream, matrix, sp, slices);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(L.H, 4096)), dim3(THREADS), 0, stream, cursor, (uint64_t)L.H);
+    hipLaunchKernelGGL(k_seg_hist, dim3(L.max_slices), dim3(THREADS), nbk * 4, stream, ktmp, matrix, slices, sp, cursor);
+    hipLaunchKernelGGL(k_seg_scan, dim3(sp.nc), dim3(THREADS), nbk * 4, stream, matrix, sp, cursor);
+    hipLaunchKernelGGL(k_seg_scatter, dim3(L.max_slices), dim3(THREADS), nbk * 8, stream, ktmp, vtmp, matrix, slices, sp,
+                       cursor, keys0, vals0);
     ZKP_CHECK(hipGetLastError());
     pt.mark("segsort");
   } else {
