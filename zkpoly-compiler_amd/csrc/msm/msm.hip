@@ -22,7 +22,8 @@
 //     that the next level reduces the same way, so a bucket holding any share
 //     of the points (e.g. the scalar-1 bucket) costs log levels, not a serial walk.
 //  4. bucket reduction per window, S_w = sum_b b * B_b, in levels of running
-//     sums over segments of 8 (no serial walk over 2^(s-1) buckets).
+//     sums over segments of 2 (no serial walk over 2^(s-1) buckets; short
+//     dependent chains, since a chain of point additions is latency bound).
 //  5. host: Horner over the window sums (s doublings each) and one inversion
 //     to affine -- the serial tail runs where a modular product is ~30x lower
 //     latency than one GPU wave (measured 738 ns per dependent product).
@@ -40,9 +41,10 @@ namespace zkp {
 namespace msm {
 
 constexpr int THREADS = 256;
-constexpr uint32_t CHUNK = 32;   // sorted entries per thread at every accumulation level
-constexpr uint32_t RED_G = 8;    // bucket-reduction segment (a power of two)
-constexpr uint32_t RED_G_LOG = 3;
+constexpr uint32_t CHUNK = 32;   // sorted entries per thread at accumulation level 0
+constexpr uint32_t CHUNK_L = 8;  // (key, partial) entries per thread at the later levels
+constexpr uint32_t RED_G = 2;    // bucket-reduction segment (a power of two)
+constexpr uint32_t RED_G_LOG = 1;
 constexpr uint32_t SCALAR_BITS = 254;
 constexpr int MAX_LEVELS = 24;
 
@@ -499,9 +501,17 @@ __global__ void __launch_bounds__(THREADS)
 
 // ---------------------------------------------------------------- accumulation
 
-__device__ __forceinline__ void bucket_add(u32* buckets, uint32_t key, const G1XYZZ& sum) {
+// Closes a bucket.  Every bucket is completed at exactly one level (its whole
+// run, or all its partials, inside one chunk), so in the first part of an MSM
+// (fresh buckets hold the identity) the sum is stored, in later parts added.
+// Identity sums are skipped: slots without a partial carry a key too.
+__device__ __forceinline__ void bucket_close(u32* buckets, uint32_t key, const G1XYZZ& sum, bool fresh) {
   if (sum.is_identity()) return;
   u32* p = buckets + (size_t)key * 32;
+  if (fresh) {
+    sum.store(p);
+    return;
+  }
   const G1XYZZ cur = G1XYZZ::load(p);
   (cur.is_identity() ? sum : cur.add(sum)).store(p);
 }
@@ -512,25 +522,25 @@ __device__ __forceinline__ G1Affine fetch_point(const u32* __restrict__ points, 
   return a;
 }
 
-// One accumulation level.  Thread t owns sorted entries [t*CHUNK, ...) and
-// sums each run of equal keys, one entry per loop step in every lane (a wave
-// takes CHUNK steps whatever the run lengths).  A run closed on both sides goes to its bucket;
+// One accumulation level.  Thread t owns sorted entries [t*CH, ...) and sums
+// each run of equal keys, one entry per loop step in every lane (a wave takes
+// CH steps whatever the run lengths).  A run closed on both sides goes to its bucket;
 // the chunk's first run (if it continues from the previous chunk) and last run
 // (if it continues into the next) are emitted as (key, partial) at slots 2t and
 // 2t + 1 of the next level, identity where there is nothing to carry, so every
 // key's pieces stay contiguous.
-template <bool FROM_POINTS>
+template <bool FROM_POINTS, uint32_t CH>
 __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                 const u32* __restrict__ src, uint32_t* counts, uint32_t level,
+                                                 const u32* __restrict__ src, uint32_t* counts, uint32_t level, bool fresh,
                                                  u32* buckets, uint32_t* __restrict__ next_keys,
                                                  u32* __restrict__ next_pts) {
   const uint32_t N = counts[level];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nchunks = (N + CHUNK - 1) / CHUNK;
+  const uint32_t nchunks = (N + CH - 1) / CH;
   if (t == 0) counts[level + 1] = 2 * nchunks;
   if (t >= nchunks) return;
-  const uint32_t start = t * CHUNK;
-  const uint32_t end = start + CHUNK < N ? start + CHUNK : N;
+  const uint32_t start = t * CH;
+  const uint32_t end = start + CH < N ? start + CH : N;
   const uint32_t key_first = keys[start], key_last = keys[end - 1];
   const bool open_left = start > 0 && keys[start - 1] == key_first;
   const bool open_right = end < N && keys[end] == key_last;
@@ -553,7 +563,7 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
         if (next_pts) acc.store(next_pts + (size_t)(2 * t + 1) * 32);
         b_done = true;
       } else {
-        bucket_add(buckets, key, acc);
+        bucket_close(buckets, key, acc, fresh);
       }
       acc = G1XYZZ::identity();
       run_start = i + 1;
@@ -570,16 +580,16 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
 // Level 0: entries are (key, point index | sign << 31).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate0(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                  const u32* __restrict__ points, uint32_t* counts, u32* buckets, uint32_t* __restrict__ next_keys,
-                  u32* __restrict__ next_pts) {
-  accumulate_chunk<true>(keys, vals, points, counts, 0, buckets, next_keys, next_pts);
+                  const u32* __restrict__ points, uint32_t* counts, bool fresh, u32* buckets,
+                  uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
+  accumulate_chunk<true, CHUNK>(keys, vals, points, counts, 0, fresh, buckets, next_keys, next_pts);
 }
 
 // Level >= 1: entries are (key, XYZZ partial).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate(const uint32_t* __restrict__ keys, const u32* __restrict__ pts, uint32_t* counts, uint32_t level,
-                 u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
-  accumulate_chunk<false>(keys, nullptr, pts, counts, level, buckets, next_keys, next_pts);
+                 bool fresh, u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
+  accumulate_chunk<false, CHUNK_L>(keys, nullptr, pts, counts, level, fresh, buckets, next_keys, next_pts);
 }
 
 __global__ void __launch_bounds__(THREADS) k_zero_u32(uint32_t* p, uint64_t n) {
@@ -720,8 +730,10 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   uint64_t m = L->n_entries_max;
   int lv = 0;
   uint64_t max_even = 0, max_odd = 0;
-  while (m > CHUNK) {
-    m = 2 * ((m + CHUNK - 1) / CHUNK);
+  uint32_t chunk = CHUNK;  // level 0, then CHUNK_L
+  while (m > chunk) {
+    m = 2 * ((m + chunk - 1) / chunk);
+    chunk = CHUNK_L;
     if (lv >= MAX_LEVELS - 1) return false;
     L->level_max[lv] = (uint32_t)m;
     if (lv % 2 == 0) max_even = m > max_even ? m : max_even;
@@ -736,6 +748,7 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   int rl = 0;
   uint64_t red_max0 = 0, red_max1 = 0;
   while (M > 1) {
+    if (rl >= MAX_LEVELS) return false;
     M = (M + RED_G - 1) / RED_G;
     L->red_m[rl] = M;
     if (rl % 2 == 0) red_max0 = M > red_max0 ? M : red_max0;
@@ -814,7 +827,7 @@ static hipError_t reset_buckets(const Config& cfg, const Layout& L, char* ws, hi
 // Adds sum_i k_i P_i for one part (len points, n_precompute tables laid out
 // table-major at d_points) into the buckets.
 static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, const u32* d_points,
-                                  const u32* d_scalars, uint64_t len, hipStream_t stream, PhaseTimer& pt) {
+                                  const u32* d_scalars, uint64_t len, bool fresh, hipStream_t stream, PhaseTimer& pt) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(ws + L.hist);
   uint32_t* cursor = reinterpret_cast<uint32_t*>(ws + L.cursor);
   uint32_t* bsums = reinterpret_cast<uint32_t*>(ws + L.block_sums);
@@ -884,16 +897,16 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
   uint32_t* nk = L.levels > 0 ? reinterpret_cast<uint32_t*>(ws + L.lvl_keys[0]) : nullptr;
   u32* np = L.levels > 0 ? reinterpret_cast<u32*>(ws + L.lvl_pts[0]) : nullptr;
   hipLaunchKernelGGL(k_accumulate0, dim3(grid_for(nchunks0, 1u << 30)), dim3(THREADS), 0, stream, keys0, vals0,
-                     d_points, counts, buckets, nk, np);
+                     d_points, counts, fresh, buckets, nk, np);
   ZKP_CHECK(hipGetLastError());
   pt.mark("accumulate0");
   for (int lv = 0; lv < L.levels; lv++) {
     const int cur = lv % 2, nxt = 1 - cur;
     const bool last = lv == L.levels - 1;
-    const uint64_t nch = (L.level_max[lv] + CHUNK - 1) / CHUNK;
+    const uint64_t nch = (L.level_max[lv] + CHUNK_L - 1) / CHUNK_L;
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(nch, 1u << 30)), dim3(THREADS), 0, stream,
                        reinterpret_cast<const uint32_t*>(ws + L.lvl_keys[cur]),
-                       reinterpret_cast<const u32*>(ws + L.lvl_pts[cur]), counts, (uint32_t)(lv + 1), buckets,
+                       reinterpret_cast<const u32*>(ws + L.lvl_pts[cur]), counts, (uint32_t)(lv + 1), fresh, buckets,
                        last ? nullptr : reinterpret_cast<uint32_t*>(ws + L.lvl_keys[nxt]),
                        last ? nullptr : reinterpret_cast<u32*>(ws + L.lvl_pts[nxt]));
     ZKP_CHECK(hipGetLastError());
@@ -951,7 +964,7 @@ static hipError_t msm_device(const Config& cfg, const u32* d_points, const u32* 
                              const Layout& L, hipStream_t stream, G1XYZZ* result, bool debug) {
   PhaseTimer pt{debug, stream, {}, {}};
   ZKP_CHECK(reset_buckets(cfg, L, ws, stream));
-  if (len > 0) ZKP_CHECK(accumulate_part(cfg, L, ws, d_points, d_scalars, len, stream, pt));
+  if (len > 0) ZKP_CHECK(accumulate_part(cfg, L, ws, d_points, d_scalars, len, true, stream, pt));
   const u32* d_sums = nullptr;
   ZKP_CHECK(reduce_buckets(cfg, L, ws, stream, &d_sums, pt));
   std::vector<G1XYZZ> sums;
@@ -1037,7 +1050,7 @@ hipError_t zkp_msm_host(unsigned int window, unsigned int target_windows, int de
         }
         if (err == hipSuccess)
           err = hipMemcpyAsync(d_sc, h_scaler_batch[b] + p_off * 8, p_len * 32, hipMemcpyHostToDevice, stream);
-        if (err == hipSuccess) err = accumulate_part(cfg, L, ws, d_pts, d_sc, p_len, stream, pt);
+        if (err == hipSuccess) err = accumulate_part(cfg, L, ws, d_pts, d_sc, p_len, p == 0, stream, pt);
       }
       const u32* d_sums = nullptr;
       if (err == hipSuccess) err = reduce_buckets(cfg, L, ws, stream, &d_sums, pt);
