@@ -1,0 +1,112 @@
+"""HBM traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a plain copy.
+
+`python -m zkpoly_amd.traffic run` launches a streaming copy of COPY_BYTES (read
+COPY_BYTES, write COPY_BYTES) and then `ntts` forward NTTs of 2^log_n through
+ssip_ntt; run it once under `rocprofv3 --pmc FETCH_SIZE` and once under
+`rocprofv3 --pmc WRITE_SIZE` (one counter set per run).
+
+`python -m zkpoly_amd.traffic summarize FETCH.csv WRITE.csv OUT.json OUT.md`
+averages each counter per kernel, scales it by (known copy bytes) / (what the
+counter reads for the copy), and writes bytes per launch.  A kernel whose
+traffic is well above its algorithmic bytes re-reads data.
+"""
+from __future__ import annotations
+
+import csv
+import json
+import sys
+from collections import defaultdict
+
+COPY_BYTES = 512 << 20
+COPY_KERNEL = "copy_u4_kernel"
+
+
+def run(log_n: int = 24, ntts: int = 3) -> None:
+    import ctypes
+
+    from . import hip
+    from .field import mont_limbs, omega, random_elements
+    from .ntt import gen_powers, load_lib
+    from .runtime import Libs, PolyPtr
+
+    libs = Libs()
+    stream = hip.Stream(0)
+    lib = load_lib(libs)
+    lib.zkp_copy_u4.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_void_p]
+    lib.zkp_copy_u4.restype = ctypes.c_int
+    src = hip.DeviceBuffer(COPY_BYTES)
+    dst = hip.DeviceBuffer(COPY_BYTES)
+    src.zero()
+    hip.check(lib.zkp_copy_u4(dst.addr, src.addr, COPY_BYTES // 16, stream.raw()), "zkp_copy_u4")
+    stream.sync()
+    del src, dst
+    n = 1 << log_n
+    x = hip.DeviceBuffer(n * 32)
+    x.upload(random_elements(n, 99))
+    w = hip.DeviceBuffer(32)
+    w.upload(mont_limbs(omega(log_n)))
+    tw = hip.DeviceBuffer(n // 2 * 32)
+    gen_powers(libs, tw.addr, n // 2, w.addr, stream)
+    ptr = PolyPtr(x.addr, n, 0, 0, n)
+    for _ in range(ntts):
+        hip.check(lib.ssip_ntt(ptr, tw.addr, log_n, stream.raw(), 8, 8), "ssip_ntt")
+    stream.sync()
+    print(f"traffic workload done: copy {COPY_BYTES} B, {ntts} NTTs of 2^{log_n}", flush=True)
+
+
+def _per_kernel(path: str, counter: str) -> dict:
+    sums, counts = defaultdict(float), defaultdict(set)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].split("::")[-1]
+            sums[name] += float(r["Counter_Value"])
+            counts[name].add(r["Dispatch_Id"])
+    return {k: sums[k] / len(counts[k]) for k in sums}
+
+
+def summarize(fetch_csv: str, write_csv: str, out_json: str, out_md: str, log_n: int = 24) -> dict:
+    fetch = _per_kernel(fetch_csv, "FETCH_SIZE")
+    write = _per_kernel(write_csv, "WRITE_SIZE")
+    f_fetch = COPY_BYTES / (fetch[COPY_KERNEL] * 1024)
+    f_write = COPY_BYTES / (write[COPY_KERNEL] * 1024)
+    alg = 64 * (1 << log_n)  # one read and one write of every 32-byte element
+    out = {"copy_bytes": COPY_BYTES, "fetch_scale": round(f_fetch, 4), "write_scale": round(f_write, 4)}
+    rows = []
+    for k in sorted(set(fetch) | set(write)):
+        fb = fetch.get(k, 0.0) * 1024 * f_fetch
+        wb = write.get(k, 0.0) * 1024 * f_write
+        out[k] = int(fb + wb)
+        rows.append((k, fetch.get(k, 0.0), write.get(k, 0.0), fb, wb))
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    with open(out_md, "w") as f:
+        f.write(f"# HBM traffic per launch (NTT 2^{log_n}, rocprofv3 FETCH_SIZE / WRITE_SIZE)\n\n")
+        f.write("Commands (repository root on the GPU box; one counter per run):\n\n```\n")
+        f.write("cd /tmp && export TMPDIR=/tmp && export PYTHONPATH=$REPO/zkpoly-compiler_amd\n")
+        for c, d in (("FETCH_SIZE", "pmc_fetch"), ("WRITE_SIZE", "pmc_write")):
+            f.write(f"rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/{d} -o run -- "
+                    "python3 -u -m zkpoly_amd.traffic run\n")
+        f.write("python3 -m zkpoly_amd.traffic summarize $REPO/gpurun_out/pmc_fetch/run_counter_collection.csv "
+                "$REPO/gpurun_out/pmc_write/run_counter_collection.csv profiles/pmc_traffic.json "
+                "profiles/pmc_traffic_ntt_2e24.md\n```\n\n")
+        f.write(f"Calibration: the copy kernel moves {COPY_BYTES} B each way; the counters read "
+                f"{fetch[COPY_KERNEL]:.0f} KiB fetched and {write[COPY_KERNEL]:.0f} KiB written for it, so they "
+                f"are scaled by {f_fetch:.4f} (fetch) and {f_write:.4f} (write).\n\n")
+        f.write("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | fetched B (scaled) | written B (scaled) | "
+                "total / algorithmic 64n B |\n|---|---|---|---|---|---|\n")
+        for k, fk, wk, fb, wb in rows:
+            ratio = (fb + wb) / alg if k != COPY_KERNEL else (fb + wb) / (2 * COPY_BYTES)
+            f.write(f"| `{k}` | {fk:.0f} | {wk:.0f} | {fb:.4g} | {wb:.4g} | {ratio:.3f} |\n")
+    return out
+
+
+if __name__ == "__main__":
+    if len(sys.argv) >= 2 and sys.argv[1] == "run":
+        run()
+    elif len(sys.argv) >= 6 and sys.argv[1] == "summarize":
+        print(json.dumps(summarize(*sys.argv[2:6]), indent=1))
+    else:
+        print(__doc__)
+        sys.exit(2)
