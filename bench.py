@@ -370,6 +370,17 @@ def traffic_from_profiles(kernel_key):
         return None
 
 
+def valu_from_profiles(kernels):
+    """VALUBusy / VALUUtilization per kernel from the committed rocprofv3 summary, if present."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return {k: d[k] for k in kernels if k in d}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -449,6 +460,7 @@ def main():
             "frac": round(int_rate / INT_MUL_PEAK, 4),
         },
         "bitrev_head_avg_ms": round(sum(per_kind[0]) / max(1, len(per_kind[0])), 5),
+        "valu_rocprof": valu_from_profiles(["dit_pass_r4", "bitrev_head", "k_accumulate0", "k_reduce_level"]),
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -9,6 +9,11 @@ ssip_ntt; run it once under `rocprofv3 --pmc FETCH_SIZE` and once under
 averages each counter per kernel, scales it by (known copy bytes) / (what the
 counter reads for the copy), and writes bytes per launch.  A kernel whose
 traffic is well above its algorithmic bytes re-reads data.
+
+`python -m zkpoly_amd.traffic valu NTT.csv MSM.csv OUT.json OUT.md` averages
+rocprofv3's derived VALUBusy (percent of time the vector ALUs issue) and
+VALUUtilization (percent of active lanes) per kernel, from runs of
+`rocprofv3 --pmc VALUBusy VALUUtilization` over bench.py's NTT and MSM legs.
 """
 from __future__ import annotations
 
@@ -102,11 +107,43 @@ def summarize(fetch_csv: str, write_csv: str, out_json: str, out_md: str, log_n:
     return out
 
 
+def summarize_valu(ntt_csv: str, msm_csv: str, out_json: str, out_md: str) -> dict:
+    out = {}
+    lines = []
+    for label, path in (("NTT 2^24 (bench.py --quick)", ntt_csv), ("MSM 2^20 (bench.py --msm-only)", msm_csv)):
+        busy = _per_kernel(path, "VALUBusy")
+        util = _per_kernel(path, "VALUUtilization")
+        for k in sorted(busy, key=lambda k: -busy[k]):
+            if k.startswith("__amd"):
+                continue
+            out[k] = {"valu_busy_pct": round(busy[k], 2), "valu_utilization_pct": round(util.get(k, 0.0), 2)}
+            lines.append(f"| {label} | `{k}` | {busy[k]:.1f} | {util.get(k, 0.0):.1f} |")
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    with open(out_md, "w") as f:
+        f.write("# VALU busy / utilisation per kernel (rocprofv3 derived counters)\n\n")
+        f.write("Commands (repository root on the GPU box):\n\n```\ncd /tmp && export TMPDIR=/tmp\n")
+        f.write("rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d $REPO/gpurun_out/pmc_valu_ntt -o run "
+                "-- python3 -u $REPO/bench.py --quick --no-cpu --steps 3 --warmup 1\n")
+        f.write("rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d $REPO/gpurun_out/pmc_valu_msm -o run "
+                "-- python3 -u $REPO/bench.py --msm-only --no-cpu --steps 2 --warmup 1\n")
+        f.write("python3 -m zkpoly_amd.traffic valu $REPO/gpurun_out/pmc_valu_ntt/run_counter_collection.csv "
+                "$REPO/gpurun_out/pmc_valu_msm/run_counter_collection.csv profiles/pmc_valu.json "
+                "profiles/pmc_valu.md\n```\n\n")
+        f.write("VALUBusy: percent of the kernel's time the vector ALUs issue instructions; VALUUtilization: percent "
+                "of lanes active in those instructions (divergence).  The MSM row averages windows 12..18.\n\n")
+        f.write("| workload | kernel | VALUBusy % | VALUUtilization % |\n|---|---|---|---|\n")
+        f.write("\n".join(lines) + "\n")
+    return out
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         run()
     elif len(sys.argv) >= 6 and sys.argv[1] == "summarize":
         print(json.dumps(summarize(*sys.argv[2:6]), indent=1))
+    elif len(sys.argv) >= 6 and sys.argv[1] == "valu":
+        print(json.dumps(summarize_valu(*sys.argv[2:6]), indent=1))
     else:
         print(__doc__)
         sys.exit(2)
