@@ -114,10 +114,13 @@ def time_loop(stream, fn, steps, warmup):
 
 def headline(libs, stream, args):
     """K timed forward NTTs of 2^log_n with per-kernel events."""
-    from zkpoly_amd import hip
+    from zkpoly_amd import autotune, hip
     from zkpoly_amd.runtime import PolyPtr  # noqa: F401
 
+    # the project's autotuner picks the pass depth for this size (untimed, before the run)
+    tuning = autotune.autotune(libs, stream, ntt_sizes=[args.log_n])
     w = Ntt(libs, stream, args.log_n, seed=2024)
+    w.tuning = tuning
     lib = w.lib
     lib.zkp_ntt_with_events.argtypes = [
         type(w.ptr),
@@ -439,6 +442,7 @@ def main():
             "workload": f"in-place forward NTT of 2^{k} BN254 Fr elements (natural order), ssip_ntt C ABI",
             "log_n": k,
             "kernels_per_step": nkern,
+            "autotuned_min_col_log": w.tuning.ntt.get(k),
         },
         "roofline": {
             "bound": "hbm",

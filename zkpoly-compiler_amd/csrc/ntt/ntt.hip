@@ -343,7 +343,64 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
   // w^0 is 1 in every table built as powers; the product by it is skipped
   // only when the table holds exactly 1, so any other table is still honoured
   const bool unit0 = lds_get(L.r0, L.r1, 0) == Fr::one();
-  for (uint32_t m = 0; m < t; m++) {
+  // stages two at a time: quads of rows {k0, k0+h, k0+2h, k0+3h} (bits m, m+1
+  // of k0 clear) of one column, two quads per thread
+  uint32_t m = 0;
+  for (; m + 1 < t; m += 2) {
+    const uint32_t h = 1u << m;
+    const uint32_t per_tile_q = W * (W >> 2);
+    const uint32_t totq = (pair ? 2 : 1) * per_tile_q;
+    const bool skip1 = m == 0 && unit0;
+    for (uint32_t u0 = threadIdx.x; u0 < totq; u0 += 2 * THREADS) {
+      Fr x[2][4];
+      uint32_t pos[2][4], jw[2];
+      bool live[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const uint32_t uq = u0 + u * THREADS;
+        live[u] = uq < totq;
+        const uint32_t uu = live[u] ? uq : u0;
+        const uint32_t tile = uu >= per_tile_q ? 1 : 0;
+        const uint32_t r = uu - tile * per_tile_q;
+        const uint32_t a = r >> (t - 2);
+        const uint32_t q = r & ((W >> 2) - 1);
+        const uint32_t k0 = ((q >> m) << (m + 2)) | (q & (h - 1));
+        const uint32_t off = tile ? 0 : y0;
+        const uint32_t ra = rev_bits(a, t);
+        jw[u] = k0 & (h - 1);
+#pragma unroll
+        for (int r4 = 0; r4 < 4; r4++) {
+          pos[u][r4] = off + rev_bits(k0 + r4 * h, t) * HEAD_PITCH + ra;
+          x[u][r4] = lds_get(L.p0, L.p1, pos[u][r4]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        Fr t1 = x[u][1], t3 = x[u][3];
+        if (!skip1) {
+          const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
+          t1 = t1 * w1;
+          t3 = t3 * w1;
+        }
+        const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
+        const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
+        const Fr t2 = a2 * lds_get(L.r0, L.r1, jw[u] << (t - 2 - m));
+        const Fr t4 = a3 * lds_get(L.r0, L.r1, (jw[u] + h) << (t - 2 - m));
+        x[u][0] = a0 + t2;
+        x[u][2] = a0 - t2;
+        x[u][1] = a1 + t4;
+        x[u][3] = a1 - t4;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!live[u]) continue;
+#pragma unroll
+        for (int r4 = 0; r4 < 4; r4++) lds_put(L.p0, L.p1, pos[u][r4], x[u][r4]);
+      }
+    }
+    __syncthreads();
+  }
+  if (m < t) {  // odd last stage, radix-2
     const uint32_t half = 1u << m;
     const bool skip = m == 0 && unit0;
     for (uint32_t u0 = threadIdx.x; u0 < total; u0 += 4 * THREADS) {
