@@ -12,6 +12,13 @@ def test_tune_keeps_argmin_per_size_and_all_timings():
     t = at.tune("ntt", [20, 24], [1, 2, 3], lambda k, c: table[(k, c)])
     assert t.ntt == {20: 2, 24: 1}
     assert t.timings["ntt/24/1"] == 3.4 and len(t.timings) == 6
+
+
+def test_tune_keeps_each_candidates_best_round():
+    """A candidate measured slowly once (clock ramp-up) still wins on its best round."""
+    seq = iter([9.0, 3.3, 3.4, 3.1, 3.3, 3.4])  # round 1: c1 slow; round 2: c1 fastest
+    t = at.tune("ntt", [24], [1, 2, 3], lambda k, c: next(seq))
+    assert t.ntt == {24: 1} and t.timings["ntt/24/1"] == 3.1
     m = at.tune("msm", [20], [14, 16], lambda k, w: {14: 7.0, 16: 6.5}[w])
     assert m.msm_window(20) == 16 and m.msm_window(18, default=13) == 13
 

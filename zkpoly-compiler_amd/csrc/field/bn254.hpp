@@ -167,6 +167,61 @@ struct Fp {
 
   ZKP_HD Fp dbl() const { return *this + *this; }
 
+  // ---- lazy forms for butterflies: values kept in [0, 4p) (4p < 2^256) ----
+  // limb i of 2p
+  static constexpr u32 p2_limb(int i) { return (Params::P[i] << 1) | (i ? Params::P[i - 1] >> 31 : 0u); }
+
+  // x - 2p if x >= 2p, for x < 4p: result in [0, 2p).
+  ZKP_HD static Fp reduce_2p(const Fp& x) {
+    Fp d;
+    u64 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      u64 t = (u64)x.v[i] - p2_limb(i) - borrow;
+      d.v[i] = (u32)t;
+      borrow = (t >> 32) & 1;
+    }
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = borrow ? x.v[i] : d.v[i];
+    return r;
+  }
+  // [0, 4p) -> canonical.
+  ZKP_HD static Fp reduce_4p(const Fp& x) { return reduce_once(reduce_2p(x)); }
+  // a + b without reduction (the caller keeps the sum below 2^256).
+  ZKP_HD static Fp add_raw(const Fp& a, const Fp& b) {
+    Fp s;
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      c = (u64)a.v[i] + b.v[i] + (c >> 32);
+      s.v[i] = (u32)c;
+    }
+    return s;
+  }
+  // a + 2p - b for a, b < 2p: result in (0, 4p).
+  ZKP_HD static Fp sub_2p(const Fp& a, const Fp& b) {
+    Fp d;
+    u64 c = 0, borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const u64 t = (u64)p2_limb(i) - b.v[i] - borrow;  // 2p - b >= 0
+      borrow = (t >> 32) & 1;
+      c = (u64)a.v[i] + (u32)t + (c >> 32);
+      d.v[i] = (u32)c;
+    }
+    return d;
+  }
+  // Montgomery product without the final subtraction: for a < 4p and b < p
+  // the result a*b*R^-1 (mod p) lies in [0, 2p).
+  ZKP_HD static Fp mul_lazy(const Fp& a, const Fp& b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return mul_ps_raw(a, b);
+#else
+    return mul_cios64_raw(a, b);
+#endif
+  }
+
   // Montgomery product a*b*R^-1 mod p.
   //
   // Device: product scanning (column-wise) with interleaved reduction.  Each
@@ -193,7 +248,8 @@ struct Fp {
     return (u64)0 - x;
   }
   // Host CIOS on 4 x 64-bit limbs (same memory layout as 8 x 32).
-  static Fp mul_cios64(const Fp& a, const Fp& b) {
+  static Fp mul_cios64(const Fp& a, const Fp& b) { return reduce_once(mul_cios64_raw(a, b)); }
+  static Fp mul_cios64_raw(const Fp& a, const Fp& b) {
     typedef unsigned __int128 u128;
     u64 A[4], B[4], P[4], t[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < 4; i++) {
@@ -228,7 +284,7 @@ struct Fp {
       r.v[2 * i] = (u32)t[i];
       r.v[2 * i + 1] = (u32)(t[i] >> 32);
     }
-    return reduce_once(r);
+    return r;
   }
 #endif
 
@@ -271,7 +327,8 @@ struct Fp {
         : "v"(a), "v"(b)
         : "vcc");
   }
-  __device__ __forceinline__ static Fp mul_ps(const Fp& A, const Fp& B) {
+  __device__ __forceinline__ static Fp mul_ps(const Fp& A, const Fp& B) { return reduce_once(mul_ps_raw(A, B)); }
+  __device__ __forceinline__ static Fp mul_ps_raw(const Fp& A, const Fp& B) {
     const u32* a = A.v;
     const u32* b = B.v;
     u32 m[8];
@@ -307,7 +364,7 @@ struct Fp {
       acc = (acc >> 32) | ((u64)c2 << 32);
     }
     t[7] = (u32)acc;
-    return reduce_once(from_raw(t));
+    return from_raw(t);
   }
 #endif
 

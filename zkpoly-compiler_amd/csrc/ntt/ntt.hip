@@ -177,7 +177,8 @@ struct BatchInstance {
   uint32_t head_t;  // stages fused into the bit reversal (0: none)
 };
 
-// Pass parameters of one instance in one round (d == 0: no pass this round).
+// Pass parameters of one instance in one round (d == 0: no pass this round);
+// pad != 0 marks the instance's last pass (canonical output).
 struct BatchPass {
   uint32_t s0, d, c_log, pad;
 };
@@ -500,9 +501,11 @@ __device__ __forceinline__ uint64_t tile_index(uint64_t base, uint64_t cstride, 
   return base + c * cstride + (uint64_t)k * kstride;
 }
 
+// Butterflies are lazy (Harvey): values stay in [0, 4p) between stages and
+// passes; the transform's last pass (final) stores canonical values.
 __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
                                             const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
-                                            uint32_t c_log, uint64_t tile) {
+                                            uint32_t c_log, uint64_t tile, bool final_pass) {
   const uint32_t C = 1u << c_log;
   const uint32_t R = 1u << d;
   const uint32_t T = R << c_log;
@@ -567,16 +570,18 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
       }
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        const Fr t1 = x[u][1] * w1[u];
-        const Fr t3 = x[u][3] * w1[u];
-        const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
-        const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
-        const Fr t2 = a2 * w2[u];
-        const Fr t4 = a3 * w3[u];
-        x[u][0] = a0 + t2;
-        x[u][2] = a0 - t2;
-        x[u][1] = a1 + t4;
-        x[u][3] = a1 - t4;
+        const Fr t1 = Fr::mul_lazy(x[u][1], w1[u]);
+        const Fr t3 = Fr::mul_lazy(x[u][3], w1[u]);
+        const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
+        const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
+        const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
+        const Fr t2 = Fr::mul_lazy(a2, w2[u]);
+        const Fr t4 = Fr::mul_lazy(a3, w3[u]);
+        const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
+        x[u][0] = Fr::add_raw(a0n, t2);
+        x[u][2] = Fr::sub_2p(a0n, t2);
+        x[u][1] = Fr::add_raw(a1n, t4);
+        x[u][3] = Fr::sub_2p(a1n, t4);
       }
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -599,11 +604,10 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
       const uint32_t pa = (ka << c_log) | c;
       const uint32_t pb = ((ka + half) << c_log) | c;
       const Fr w = Fr::load(tw + (((tile_index(base, cstride, kstride, c, ka) & jmask) << tw_shift) << 3));
-      const Fr x = lds_get(lds0, lds1, pa);
-      const Fr y = lds_get(lds0, lds1, pb);
-      const Fr t = y * w;
-      lds_put(lds0, lds1, pa, x + t);
-      lds_put(lds0, lds1, pb, x - t);
+      const Fr x = Fr::reduce_2p(lds_get(lds0, lds1, pa));
+      const Fr t = Fr::mul_lazy(lds_get(lds0, lds1, pb), w);
+      lds_put(lds0, lds1, pa, Fr::add_raw(x, t));
+      lds_put(lds0, lds1, pb, Fr::sub_2p(x, t));
     }
     __syncthreads();
   }
@@ -619,16 +623,23 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
     }
     uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
     const uint32_t pos = (k << c_log) | c;
-    dst[0] = lds0[pos];
-    dst[1] = lds1[pos];
+    if (final_pass) {
+      const Fr y = Fr::reduce_4p(lds_get(lds0, lds1, pos));
+      dst[0] = make_uint4(y.v[0], y.v[1], y.v[2], y.v[3]);
+      dst[1] = make_uint4(y.v[4], y.v[5], y.v[6], y.v[7]);
+    } else {
+      dst[0] = lds0[pos];
+      dst[1] = lds1[pos];
+    }
   }
 }
 
 __global__ void __launch_bounds__(THREADS)
-    dit_pass_r4(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log) {
+    dit_pass_r4(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log,
+                uint32_t final_pass) {
   __shared__ uint4 lds0[1 << TILE_LOG];
   __shared__ uint4 lds1[1 << TILE_LOG];
-  dit_tile_r4(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x);
+  dit_tile_r4(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x, final_pass != 0);
 }
 
 __global__ void __launch_bounds__(THREADS)
@@ -640,7 +651,7 @@ __global__ void __launch_bounds__(THREADS)
   if (b >= tile_prefix[count]) return;
   const uint32_t k = find_instance(tile_prefix, count, b);
   const BatchPass p = pass[k];
-  dit_tile_r4(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k]);
+  dit_tile_r4(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k], p.pad != 0);
 }
 
 // ---------------------------------------------------------------- recompute NTT
@@ -991,7 +1002,8 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
     for (int p = 0; p < plan.np; p++) {
       hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)plan.pass[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
-                         (uint32_t)log_len, plan.pass[p].s0, plan.pass[p].d, plan.pass[p].c_log);
+                         (uint32_t)log_len, plan.pass[p].s0, plan.pass[p].d, plan.pass[p].c_log,
+                         (uint32_t)(p + 1 == plan.np));
       if ((err = hipGetLastError()) != hipSuccess || (err = mark(1)) != hipSuccess) return err;
     }
   }
@@ -1284,7 +1296,7 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
       uint64_t tiles = 0;
       if (r < plans[i].size()) {
         const PassPlan& q = plans[i][r];
-        passes[i] = BatchPass{q.s0, q.d, q.c_log, 0};
+        passes[i] = BatchPass{q.s0, q.d, q.c_log, (uint32_t)(r + 1 == plans[i].size())};
         tiles = q.tiles;
       }
       tile_prefix[i + 1] = tile_prefix[i] + tiles;

@@ -100,16 +100,20 @@ def pick(times: dict) -> object:
     return best
 
 
-def tune(kind: str, sizes: Iterable[int], candidates: Iterable[int], measure: Callable[[int, int], float]) -> Tuning:
-    """Times measure(size, candidate) for every pair and keeps the argmin per size."""
+def tune(kind: str, sizes: Iterable[int], candidates: Iterable[int], measure: Callable[[int, int], float],
+         rounds: int = 2) -> Tuning:
+    """Times measure(size, candidate) for every pair in `rounds` interleaved
+    rounds (so clock ramp-up or a noisy moment does not favour one candidate)
+    and keeps the argmin of each candidate's best time per size."""
     out = Tuning()
     cands = list(candidates)
     for size in sizes:
-        times = {}
+        times = {c: float("inf") for c in cands}
+        for _ in range(max(1, rounds)):
+            for c in cands:
+                times[c] = min(times[c], float(measure(size, c)))
         for c in cands:
-            ms = float(measure(size, c))
-            times[c] = ms
-            out.timings[f"{kind}/{size}/{c}"] = round(ms, 5)
+            out.timings[f"{kind}/{size}/{c}"] = round(times[c], 5)
         (out.ntt if kind == "ntt" else out.msm)[size] = pick(times)
     return out
 
@@ -128,7 +132,7 @@ def _device_name() -> str:
 class NttTimer:
     """measure(log_n, min_col_log): mean ms of ssip_ntt on a resident 2^log_n input."""
 
-    def __init__(self, libs, stream, steps: int = 5, warmup: int = 2):
+    def __init__(self, libs, stream, steps: int = 5, warmup: int = 3):
         self.libs, self.stream, self.steps, self.warmup = libs, stream, steps, warmup
         self._cache = {}
 
