@@ -70,8 +70,10 @@ hipError_t zkp_ntt_batch_run(void* plan, hipStream_t stream);
 void zkp_ntt_batch_destroy(void* plan);
 
 /* Transform variant for later calls and batch plans in this process: 0 = bit
- * reversal fused with the first stages + radix-4 passes (default), 1 = plain
- * bit reversal + radix-2 passes.  Returns the previous variant, -1 if unknown. */
+ * reversal fused with the first stages (persistent workgroups) + radix-4 passes
+ * (default), 1 = plain bit reversal + radix-2 passes, 2 = as 0 with one tile
+ * pair per head workgroup (batch plans treat 2 as 0).  Returns the previous
+ * variant, -1 if unknown. */
 int zkp_ntt_set_variant(int variant);
 
 /* Autotuner hook: passes of 2^log_n transforms read runs of >= 2^min_col_log

@@ -19,15 +19,18 @@
 // table the reference's compiler stores as a constant).
 //
 // Kernels (default plan):
-//  * bitrev_head: the bit-reversal permutation fused with DIT stages 0..4; a
-//    workgroup swaps a pair of 32 x 32 tiles through LDS and runs the five
-//    stages on them before writing back.
+//  * bitrev_head_stream: the bit-reversal permutation fused with DIT stages
+//    0..4; a workgroup swaps a pair of 32 x 32 tiles through LDS and runs the
+//    five stages on them before writing back, walking its share of the tile
+//    pairs with the next pair's loads in flight during the current stages.
 //  * dit_pass_r4: a workgroup loads a tile of 2^TILE_LOG elements -- 2^d
 //    butterfly-connected rows times 2^c adjacent columns, so every global
 //    access is a run of >= 2^c consecutive 32-byte elements -- into LDS (two
 //    16-byte planes), runs d consecutive stages there two at a time (radix-4
 //    register blocks), and writes it back.
-//    A transform of 2^24 is bitrev_head + 3 passes (7 + 6 + 6 stages), or
+//    Between stages and passes values stay in [0, 4p) (lazy reduction); the
+//    last pass stores canonical values.
+//    A transform of 2^24 is the head + 3 passes (7 + 6 + 6 stages), or
 //    + 2 passes (10 + 9) with min_col_log = 1.
 // The first version (bitrev_kernel + radix-2 dit_pass, 11 + 9 + 4 stages) is
 // kept selectable through zkp_ntt_set_variant for comparison.
@@ -297,52 +300,58 @@ struct HeadLds {
   uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
 };
 
-// The caller has written L.r (visible after the first barrier below).
-__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint64_t M) {
+// Per-thread staging of the head's global loads: element e = j * THREADS +
+// threadIdx.x of the tile at M (a) and of the tile at rev M (b), j < 4.
+struct HeadStaged {
+  uint4 a0, a1, b0, b1;
+};
+static_assert(HEAD_LOADS == 4, "head staging is written for four loads per thread");
+
+// Load j of the thread (element e = j * THREADS + threadIdx.x of the tile at M
+// and of the tile at rev M).  The loads are unconditional: the index is
+// clamped for t < 5, and x at M is read twice when M == rev M.  Callers keep
+// the four results in named values so the staging stays in registers.
+__device__ __forceinline__ HeadStaged head_fetch(const PolyView& v, uint32_t t, uint32_t shift_a, uint64_t M,
+                                                 uint64_t Mr, uint32_t j) {
   const uint32_t W = 1u << t;
-  const uint32_t mbits = log_n - 2 * t;
-  const uint64_t Mr = mbits ? (__brevll(M) >> (64 - mbits)) : 0;
-  if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
-  const bool pair = M != Mr;
-  const uint32_t shift_a = log_n - t;
-  // all global loads of the thread first, then the LDS stores; the loads are
-  // unconditional (index clamped for t < 5, x at M read twice when M == rev M)
-  // and held in named values, so the staging stays in registers
-  struct Staged {
-    uint4 a0, a1, b0, b1;
-  };
-  auto fetch = [&](uint32_t j) {
-    const uint32_t e = (threadIdx.x + j * THREADS) & (W * W - 1);
-    const uint32_t a = e >> t, b = e & (W - 1);
-    const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
-    const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
-    return Staged{s[0], s[1], q[0], q[1]};
-  };
-  auto place = [&](uint32_t j, const Staged& g) {
-    const uint32_t e = threadIdx.x + j * THREADS;
-    if (e < W * W) {
-      const uint32_t p = (e >> t) * HEAD_PITCH + (e & (W - 1));
-      L.p0[p] = g.a0;
-      L.p1[p] = g.a1;
-      if (pair) {
-        L.p0[HEAD_BUF + p] = g.b0;
-        L.p1[HEAD_BUF + p] = g.b1;
-      }
+  const uint32_t e = (threadIdx.x + j * THREADS) & (W * W - 1);
+  const uint32_t a = e >> t, b = e & (W - 1);
+  const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
+  const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
+  return HeadStaged{s[0], s[1], q[0], q[1]};
+}
+
+// Staged load j into the two LDS tile buffers (x at M at offset 0, x at rev M
+// at HEAD_BUF); the caller puts a barrier after the four.
+__device__ __forceinline__ void head_place(HeadLds& L, uint32_t t, bool pair, uint32_t j, const HeadStaged& g) {
+  const uint32_t W = 1u << t;
+  const uint32_t e = threadIdx.x + j * THREADS;
+  if (e < W * W) {
+    const uint32_t p = (e >> t) * HEAD_PITCH + (e & (W - 1));
+    L.p0[p] = g.a0;
+    L.p1[p] = g.a1;
+    if (pair) {
+      L.p0[HEAD_BUF + p] = g.b0;
+      L.p1[HEAD_BUF + p] = g.b1;
     }
-  };
-  static_assert(HEAD_LOADS == 4, "head staging is written for four loads per thread");
-  const Staged g0 = fetch(0), g1 = fetch(1), g2 = fetch(2), g3 = fetch(3);
-  place(0, g0);
-  place(1, g1);
-  place(2, g2);
-  place(3, g3);
-  __syncthreads();
+  }
+}
+
+// DIT stages 0..t-1 on the placed tile(s), through the transposed index map;
+// ends with a barrier.  LAZY keeps values in [0, 4p) (Harvey butterflies, as in
+// the passes), for callers whose later passes accept such inputs; otherwise
+// every stage output is canonical.
+template <bool LAZY>
+__device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
+  const uint32_t W = 1u << t;
   // tile 0 = y at M (in the buffer of x at Mr), tile 1 = y at Mr (buffer of x at M)
   const uint32_t y0 = pair ? HEAD_BUF : 0;
   const uint32_t per_tile = W * (W >> 1);
   const uint32_t total = (pair ? 2 : 1) * per_tile;
   // w^0 is 1 in every table built as powers; the product by it is skipped
-  // only when the table holds exactly 1, so any other table is still honoured
+  // only when the table holds exactly 1, so any other table is still honoured.
+  // The skip happens at stage 0 only, whose inputs are the caller's canonical
+  // values, so the lazy bounds below hold for the unmultiplied operand too.
   const bool unit0 = lds_get(L.r0, L.r1, 0) == Fr::one();
   // stages two at a time: quads of rows {k0, k0+h, k0+2h, k0+3h} (bits m, m+1
   // of k0 clear) of one column, two quads per thread
@@ -378,19 +387,39 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         Fr t1 = x[u][1], t3 = x[u][3];
-        if (!skip1) {
-          const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
-          t1 = t1 * w1;
-          t3 = t3 * w1;
+        const Fr w2 = lds_get(L.r0, L.r1, jw[u] << (t - 2 - m));
+        const Fr w3 = lds_get(L.r0, L.r1, (jw[u] + h) << (t - 2 - m));
+        if (LAZY) {
+          if (!skip1) {
+            const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
+            t1 = Fr::mul_lazy(t1, w1);
+            t3 = Fr::mul_lazy(t3, w1);
+          }
+          const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
+          const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
+          const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
+          const Fr t2 = Fr::mul_lazy(a2, w2);
+          const Fr t4 = Fr::mul_lazy(a3, w3);
+          const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
+          x[u][0] = Fr::add_raw(a0n, t2);
+          x[u][2] = Fr::sub_2p(a0n, t2);
+          x[u][1] = Fr::add_raw(a1n, t4);
+          x[u][3] = Fr::sub_2p(a1n, t4);
+        } else {
+          if (!skip1) {
+            const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
+            t1 = t1 * w1;
+            t3 = t3 * w1;
+          }
+          const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
+          const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
+          const Fr t2 = a2 * w2;
+          const Fr t4 = a3 * w3;
+          x[u][0] = a0 + t2;
+          x[u][2] = a0 - t2;
+          x[u][1] = a1 + t4;
+          x[u][3] = a1 - t4;
         }
-        const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
-        const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
-        const Fr t2 = a2 * lds_get(L.r0, L.r1, jw[u] << (t - 2 - m));
-        const Fr t4 = a3 * lds_get(L.r0, L.r1, (jw[u] + h) << (t - 2 - m));
-        x[u][0] = a0 + t2;
-        x[u][2] = a0 - t2;
-        x[u][1] = a1 + t4;
-        x[u][3] = a1 - t4;
       }
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -428,13 +457,22 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
       }
       if (!skip) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) y[j] = y[j] * lds_get(L.r0, L.r1, wi[j]);
+        for (int j = 0; j < 4; j++) {
+          const Fr w = lds_get(L.r0, L.r1, wi[j]);
+          y[j] = LAZY ? Fr::mul_lazy(y[j], w) : y[j] * w;
+        }
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const Fr tt = y[j];
-        y[j] = x[j] - tt;
-        x[j] = x[j] + tt;
+        if (LAZY) {
+          const Fr xr = Fr::reduce_2p(x[j]);
+          y[j] = Fr::sub_2p(xr, tt);
+          x[j] = Fr::add_raw(xr, tt);
+        } else {
+          y[j] = x[j] - tt;
+          x[j] = x[j] + tt;
+        }
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -445,6 +483,13 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
     }
     __syncthreads();
   }
+}
+
+// Both y tiles back in place (y at M from the buffer at y0, y at Mr from 0).
+__device__ __forceinline__ void head_store(const HeadLds& L, const PolyView& v, uint32_t t, uint32_t shift_a,
+                                           uint64_t M, uint64_t Mr, bool pair) {
+  const uint32_t W = 1u << t;
+  const uint32_t y0 = pair ? HEAD_BUF : 0;
   for (uint32_t e = threadIdx.x; e < W * W; e += THREADS) {
     const uint32_t a = e >> t, b = e & (W - 1);
     const uint32_t p = rev_bits(b, t) * HEAD_PITCH + rev_bits(a, t);
@@ -459,6 +504,26 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
   }
 }
 
+// One tile pair {M, rev M} per workgroup.  The caller has written L.r
+// (visible after the first barrier below).
+template <bool LAZY>
+__device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint64_t M) {
+  const uint32_t mbits = log_n - 2 * t;
+  const uint64_t Mr = mbits ? (__brevll(M) >> (64 - mbits)) : 0;
+  if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
+  const bool pair = M != Mr;
+  const uint32_t shift_a = log_n - t;
+  const HeadStaged g0 = head_fetch(v, t, shift_a, M, Mr, 0), g1 = head_fetch(v, t, shift_a, M, Mr, 1),
+                   g2 = head_fetch(v, t, shift_a, M, Mr, 2), g3 = head_fetch(v, t, shift_a, M, Mr, 3);
+  head_place(L, t, pair, 0, g0);
+  head_place(L, t, pair, 1, g1);
+  head_place(L, t, pair, 2, g2);
+  head_place(L, t, pair, 3, g3);
+  __syncthreads();
+  head_stages<LAZY>(L, t, pair);
+  head_store(L, v, t, shift_a, M, Mr, pair);
+}
+
 // L.r[i] = tw[i << (log_n - t)] for i < 2^(t - 1)
 __device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint32_t* __restrict__ tw, uint32_t log_n,
                                                          uint32_t t) {
@@ -466,11 +531,86 @@ __device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint3
     lds_put(L.r0, L.r1, threadIdx.x, Fr::load(tw + (((uint64_t)threadIdx.x << (log_n - t)) << 3)));
 }
 
+// Persistent head: 2^g workgroups (g <= mbits = log_n - 2t, mbits - g even)
+// walk the tile pairs.  Walk index k = H : L (h = (mbits - g) / 2 bits each) of
+// workgroup b is the tile row
+//   M = H << (h + g) | b << h | L,   rev M = rev(L) << (h + g) | rev(b) << h | rev(H),
+// so the workgroup's own bits sit in the middle.  The pair {M, rev M} is taken
+// by this workgroup when H < rev(L) (M is the smaller of the two); when H ==
+// rev(L), rev M is the same walk index in workgroup rev(b), and the pair goes to
+// the smaller of b, rev(b) for even H and to the larger for odd H.  Every pair
+// is thus taken exactly once, and each workgroup gets the same number of pairs
+// up to 2^(h-1).  The loads of the next pair are issued before the stages of
+// the current one and land in registers while they run (barriers wait only for
+// LDS), so the tile traffic overlaps the butterflies.  The stages are lazy: a
+// radix-4 pass always follows the head and takes values in [0, 4p).
+__global__ void __launch_bounds__(THREADS)
+    bitrev_head_stream(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t, uint32_t g) {
+  __shared__ HeadLds L;
+  head_twiddles_from_table(L, tw, log_n, t);
+  const uint32_t mbits = log_n - 2 * t;
+  const uint32_t h = (mbits - g) >> 1;
+  const uint64_t K = 1ull << (2 * h);
+  const uint64_t lmask = (1ull << h) - 1;
+  const uint64_t b = blockIdx.x;
+  const uint64_t rb = g ? (__brevll(b) >> (64 - g)) : 0;
+  const uint32_t shift_a = log_n - t;
+  auto rev_h = [&](uint64_t x) -> uint64_t { return h ? (__brevll(x) >> (64 - h)) : 0; };
+  auto item = [&](uint64_t k) -> uint64_t { return ((k >> h) << (h + g)) | (b << h) | (k & lmask); };
+  auto rev = [&](uint64_t M) -> uint64_t { return mbits ? (__brevll(M) >> (64 - mbits)) : 0; };
+  auto owned = [&](uint64_t k) -> bool {
+    const uint64_t H = k >> h, rl = rev_h(k & lmask);
+    if (H != rl) return H < rl;
+    return b == rb || ((H & 1) ? b > rb : b < rb);
+  };
+  auto next = [&](uint64_t k) -> uint64_t {  // first owned k' >= k, or K
+    while (k < K && !owned(k)) k++;
+    return k;
+  };
+  uint64_t k = next(0);
+  if (k >= K) return;
+  HeadStaged g0, g1, g2, g3;
+  auto fetch = [&](uint64_t M) {
+    const uint64_t Mr = rev(M);
+    g0 = head_fetch(v, t, shift_a, M, Mr, 0);
+    g1 = head_fetch(v, t, shift_a, M, Mr, 1);
+    g2 = head_fetch(v, t, shift_a, M, Mr, 2);
+    g3 = head_fetch(v, t, shift_a, M, Mr, 3);
+  };
+  fetch(item(k));
+  for (;;) {
+    const uint64_t M = item(k), Mr = rev(M);
+    const bool pair = M != Mr;
+    head_place(L, t, pair, 0, g0);
+    head_place(L, t, pair, 1, g1);
+    head_place(L, t, pair, 2, g2);
+    head_place(L, t, pair, 3, g3);
+    __syncthreads();
+    const uint64_t kn = next(k + 1);
+    if (kn < K) fetch(item(kn));
+    head_stages<true>(L, t, pair);
+    head_store(L, v, t, shift_a, M, Mr, pair);
+    if (kn >= K) break;
+    __syncthreads();  // the store phase has read the tiles before they are replaced
+    k = kn;
+  }
+}
+
+// Workgroup bits of bitrev_head_stream for mbits = log_n - 2t: at most
+// HEAD_STREAM_G_MAX (1024 workgroups, two rounds of the two per CU that the
+// 68 KB of LDS allows on 256 CUs), with mbits - g even.
+constexpr uint32_t HEAD_STREAM_G_MAX = 10;
+static inline uint32_t head_stream_g(uint32_t mbits) {
+  uint32_t g = mbits < HEAD_STREAM_G_MAX ? mbits : HEAD_STREAM_G_MAX;
+  if ((mbits - g) & 1) g--;
+  return g;
+}
+
 __global__ void __launch_bounds__(THREADS)
     bitrev_head(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t) {
   __shared__ HeadLds L;
   head_twiddles_from_table(L, tw, log_n, t);
-  head_tile(L, v, log_n, t, blockIdx.x);
+  head_tile<true>(L, v, log_n, t, blockIdx.x);
 }
 
 // Fused bit reversal + first head_t stages of every instance; instance k owns
@@ -485,7 +625,7 @@ __global__ void __launch_bounds__(THREADS)
   const uint32_t k = find_instance(head_prefix, count, b);
   const BatchInstance I = inst[k];
   head_twiddles_from_table(L, I.tw, I.log_n, I.head_t);
-  head_tile(L, I.v, I.log_n, I.head_t, b - head_prefix[k]);
+  head_tile<true>(L, I.v, I.log_n, I.head_t, b - head_prefix[k]);
 }
 
 // ---------------------------------------------------------------- radix-4 pass
@@ -717,7 +857,7 @@ __global__ void __launch_bounds__(THREADS)
   const uint32_t mbits = log_n - 2 * t;
   if (mbits && M > (__brevll(M) >> (64 - mbits))) return;  // uniform per block
   build_small_roots(L.r0, L.r1, pq, pq_deg, omegas, log_n, t);
-  head_tile(L, v, log_n, t, M);
+  head_tile<false>(L, v, log_n, t, M);
 }
 
 struct RPassLds {
@@ -891,7 +1031,9 @@ struct PlanV2 {
 // set by the autotuner through zkp_ntt_set_tuning
 static uint32_t g_min_col_log[41] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
                                      2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-static int g_variant = 0;  // 0: v2 (head + radix-4), 1: v1 (bit reversal + radix-2 passes)
+// 0: persistent head + radix-4 passes; 1: plain bit reversal + radix-2 passes;
+// 2: as 0 with one tile pair per head workgroup (batch plans treat 2 as 0)
+static int g_variant = 0;
 
 static void plan_v2(uint32_t log_n, PlanV2* p) {
   uint32_t t = log_n / 2;
@@ -995,9 +1137,14 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
       hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
                          (uint32_t)log_len);
     } else {
-      const uint64_t blocks = 1ull << (log_len - 2 * plan.head_t);
-      hipLaunchKernelGGL(bitrev_head, dim3((unsigned)blocks), dim3(THREADS), 0, stream, v, twiddle, (uint32_t)log_len,
-                         plan.head_t);
+      if (g_variant == 2) {
+        hipLaunchKernelGGL(bitrev_head, dim3((unsigned)(1ull << (log_len - 2 * plan.head_t))), dim3(THREADS), 0,
+                           stream, v, twiddle, (uint32_t)log_len, plan.head_t);
+      } else {
+        const uint32_t g = head_stream_g((uint32_t)log_len - 2 * plan.head_t);
+        hipLaunchKernelGGL(bitrev_head_stream, dim3(1u << g), dim3(THREADS), 0, stream, v, twiddle,
+                           (uint32_t)log_len, plan.head_t, g);
+      }
     }
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
     for (int p = 0; p < plan.np; p++) {
@@ -1070,7 +1217,7 @@ hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned 
 // Not thread-safe against concurrent launches.  No counterpart in the
 // reference ABI (tuning hook).
 int zkp_ntt_set_variant(int variant) {
-  if (variant != 0 && variant != 1) return -1;
+  if (variant < 0 || variant > 2) return -1;
   const int old = zkp::ntt::g_variant;
   zkp::ntt::g_variant = variant;
   return old;

@@ -199,6 +199,7 @@ def gen_powers(libs: Libs, out_dev: int, length: int, base_dev: int, stream: hip
 
 VARIANT_FUSED_RADIX4 = 0
 VARIANT_RADIX2 = 1
+VARIANT_HEAD_PER_PAIR = 2  # as VARIANT_FUSED_RADIX4 with one tile pair per head workgroup
 
 
 def set_variant(libs: Libs, variant: int) -> int:
