@@ -25,6 +25,13 @@ namespace zkp {
 typedef uint32_t u32;
 typedef uint64_t u64;
 
+// 32-bit add / subtract with carry through clang's __builtin_addc and
+// __builtin_subc: the AMDGPU backend lowers a chain of them to one v_add_co /
+// v_addc (v_sub_co / v_subb) per limb, where the same chain written with
+// 64-bit sums and shifts costs three to five instructions per limb.
+ZKP_HD u32 adc(u32 a, u32 b, u32& carry) { return __builtin_addc(a, b, carry, &carry); }
+ZKP_HD u32 sbb(u32 a, u32 b, u32& borrow) { return __builtin_subc(a, b, borrow, &borrow); }
+
 struct FrParams {
   // r = 0x30644e72e131a029b85045b68181585d2833e84879b9709143e1f593f0000001
   static constexpr u32 P[8] = {0xf0000001u, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
@@ -103,13 +110,9 @@ struct Fp {
   // r = x - p if x >= p else x (x < 2p).
   ZKP_HD static Fp reduce_once(const Fp& x) {
     Fp d;
-    u64 borrow = 0;
+    u32 borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      u64 t = (u64)x.v[i] - Params::P[i] - borrow;
-      d.v[i] = (u32)t;
-      borrow = (t >> 32) & 1;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = sbb(x.v[i], Params::P[i], borrow);
     // borrow set <=> x < p
     Fp r;
 #pragma unroll
@@ -119,46 +122,32 @@ struct Fp {
 
   ZKP_HD Fp operator+(const Fp& o) const {
     Fp s;
-    u64 c = 0;
+    u32 c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      c = (u64)v[i] + o.v[i] + (c >> 32);
-      s.v[i] = (u32)c;
-    }
+    for (int i = 0; i < 8; i++) s.v[i] = adc(v[i], o.v[i], c);
     // 2p < 2^255: no carry out of limb 7
     return reduce_once(s);
   }
 
   ZKP_HD Fp operator-(const Fp& o) const {
     Fp d;
-    u64 borrow = 0;
+    u32 borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      u64 t = (u64)v[i] - o.v[i] - borrow;
-      d.v[i] = (u32)t;
-      borrow = (t >> 32) & 1;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = sbb(v[i], o.v[i], borrow);
     // add p back if the subtraction wrapped
-    u32 mask = 0u - (u32)borrow;
-    u64 c = 0;
+    const u32 mask = 0u - borrow;
+    u32 c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      c = (u64)d.v[i] + (Params::P[i] & mask) + (c >> 32);
-      d.v[i] = (u32)c;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = adc(d.v[i], Params::P[i] & mask, c);
     return d;
   }
 
   ZKP_HD Fp neg() const {
     // zero stays zero (reference field.cuh:1066-1079)
     Fp d;
-    u64 borrow = 0;
+    u32 borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      u64 t = (u64)Params::P[i] - v[i] - borrow;
-      d.v[i] = (u32)t;
-      borrow = (t >> 32) & 1;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = sbb(Params::P[i], v[i], borrow);
     u32 nz = is_zero() ? 0u : 0xffffffffu;
 #pragma unroll
     for (int i = 0; i < 8; i++) d.v[i] &= nz;
@@ -174,13 +163,9 @@ struct Fp {
   // x - 2p if x >= 2p, for x < 4p: result in [0, 2p).
   ZKP_HD static Fp reduce_2p(const Fp& x) {
     Fp d;
-    u64 borrow = 0;
+    u32 borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      u64 t = (u64)x.v[i] - p2_limb(i) - borrow;
-      d.v[i] = (u32)t;
-      borrow = (t >> 32) & 1;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = sbb(x.v[i], p2_limb(i), borrow);
     Fp r;
 #pragma unroll
     for (int i = 0; i < 8; i++) r.v[i] = borrow ? x.v[i] : d.v[i];
@@ -191,25 +176,17 @@ struct Fp {
   // a + b without reduction (the caller keeps the sum below 2^256).
   ZKP_HD static Fp add_raw(const Fp& a, const Fp& b) {
     Fp s;
-    u64 c = 0;
+    u32 c = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      c = (u64)a.v[i] + b.v[i] + (c >> 32);
-      s.v[i] = (u32)c;
-    }
+    for (int i = 0; i < 8; i++) s.v[i] = adc(a.v[i], b.v[i], c);
     return s;
   }
   // a + 2p - b for a, b < 2p: result in (0, 4p).
   ZKP_HD static Fp sub_2p(const Fp& a, const Fp& b) {
     Fp d;
-    u64 c = 0, borrow = 0;
+    u32 c = 0, borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const u64 t = (u64)p2_limb(i) - b.v[i] - borrow;  // 2p - b >= 0
-      borrow = (t >> 32) & 1;
-      c = (u64)a.v[i] + (u32)t + (c >> 32);
-      d.v[i] = (u32)c;
-    }
+    for (int i = 0; i < 8; i++) d.v[i] = adc(a.v[i], sbb(p2_limb(i), b.v[i], borrow), c);  // 2p - b >= 0
     return d;
   }
   // Montgomery product without the final subtraction: for a < 4p and b < p

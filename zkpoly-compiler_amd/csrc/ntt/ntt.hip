@@ -178,6 +178,7 @@ struct BatchInstance {
   const uint32_t* tw;
   uint32_t log_n;
   uint32_t head_t;  // stages fused into the bit reversal (0: none)
+  uint32_t head_g;  // the head's 2^head_g workgroups walk the tile pairs
 };
 
 // Pass parameters of one instance in one round (d == 0: no pass this round);
@@ -544,15 +545,13 @@ __device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint3
 // the current one and land in registers while they run (barriers wait only for
 // LDS), so the tile traffic overlaps the butterflies.  The stages are lazy: a
 // radix-4 pass always follows the head and takes values in [0, 4p).
-__global__ void __launch_bounds__(THREADS)
-    bitrev_head_stream(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t, uint32_t g) {
-  __shared__ HeadLds L;
-  head_twiddles_from_table(L, tw, log_n, t);
+// Workgroup b's walk (b < 2^g); the caller has written L.r.
+__device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint32_t g,
+                                          uint64_t b) {
   const uint32_t mbits = log_n - 2 * t;
   const uint32_t h = (mbits - g) >> 1;
   const uint64_t K = 1ull << (2 * h);
   const uint64_t lmask = (1ull << h) - 1;
-  const uint64_t b = blockIdx.x;
   const uint64_t rb = g ? (__brevll(b) >> (64 - g)) : 0;
   const uint32_t shift_a = log_n - t;
   auto rev_h = [&](uint64_t x) -> uint64_t { return h ? (__brevll(x) >> (64 - h)) : 0; };
@@ -596,6 +595,13 @@ __global__ void __launch_bounds__(THREADS)
   }
 }
 
+__global__ void __launch_bounds__(THREADS)
+    bitrev_head_stream(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t, uint32_t g) {
+  __shared__ HeadLds L;
+  head_twiddles_from_table(L, tw, log_n, t);
+  head_walk(L, v, log_n, t, g, blockIdx.x);
+}
+
 // Workgroup bits of bitrev_head_stream for mbits = log_n - 2t: at most
 // HEAD_STREAM_G_MAX (1024 workgroups, two rounds of the two per CU that the
 // 68 KB of LDS allows on 256 CUs), with mbits - g even.
@@ -614,8 +620,8 @@ __global__ void __launch_bounds__(THREADS)
 }
 
 // Fused bit reversal + first head_t stages of every instance; instance k owns
-// blocks [head_prefix[k], head_prefix[k + 1]) (2^(log_n - 2 head_t) of them, or
-// none when head_t == 0).
+// blocks [head_prefix[k], head_prefix[k + 1]) (2^head_g of them walking its tile
+// pairs as in bitrev_head_stream, or none when head_t == 0).
 __global__ void __launch_bounds__(THREADS)
     bitrev_head_batched(const BatchInstance* __restrict__ inst, const uint64_t* __restrict__ head_prefix,
                         uint32_t count) {
@@ -625,7 +631,7 @@ __global__ void __launch_bounds__(THREADS)
   const uint32_t k = find_instance(head_prefix, count, b);
   const BatchInstance I = inst[k];
   head_twiddles_from_table(L, I.tw, I.log_n, I.head_t);
-  head_tile<true>(L, I.v, I.log_n, I.head_t, b - head_prefix[k]);
+  head_walk(L, I.v, I.log_n, I.head_t, I.head_g, b - head_prefix[k]);
 }
 
 // ---------------------------------------------------------------- radix-4 pass
@@ -1406,7 +1412,7 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
       delete plan;
       return hipErrorInvalidValue;
     }
-    inst[i] = BatchInstance{v, twiddles[i], log_lens[i], 0};
+    inst[i] = BatchInstance{v, twiddles[i], log_lens[i], 0, 0};
     if (log_lens[i] == 0) {
       elem_prefix[i + 1] = elem_prefix[i];
       continue;
@@ -1420,8 +1426,9 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
       PlanV2 p2;
       plan_v2(log_lens[i], &p2);
       inst[i].head_t = p2.head_t;
+      inst[i].head_g = p2.head_t ? head_stream_g(log_lens[i] - 2 * p2.head_t) : 0;
       // head_t == 0 only for log_n == 1, whose bit reversal is the identity
-      elem_prefix[i + 1] = elem_prefix[i] + (p2.head_t ? (1ull << (log_lens[i] - 2 * p2.head_t)) : 0);
+      elem_prefix[i + 1] = elem_prefix[i] + (p2.head_t ? (1ull << inst[i].head_g) : 0);
       plans[i].assign(p2.pass, p2.pass + p2.np);
     }
     if (plans[i].size() > rounds) rounds = plans[i].size();
