@@ -198,6 +198,51 @@ struct Fp {
     return mul_cios64_raw(a, b);
 #endif
   }
+  // Two independent lazy products (device: interleaved chains).
+  ZKP_HD static void mul_lazy2(const Fp& a0, const Fp& b0, const Fp& a1, const Fp& b1, Fp& r0, Fp& r1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const Fp a[2] = {a0, a1}, b[2] = {b0, b1};
+    Fp r[2];
+    mul_ps_raw_n<2>(a, b, r);
+    r0 = r[0];
+    r1 = r[1];
+#else
+    r0 = mul_cios64_raw(a0, b0);
+    r1 = mul_cios64_raw(a1, b1);
+#endif
+  }
+  // N independent canonical products r[n] = a[n] * b[n].
+  template <int N>
+  ZKP_HD static void muln(const Fp (&a)[N], const Fp (&b)[N], Fp (&r)[N]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    mul_ps_raw_n<N>(a, b, r);
+#pragma unroll
+    for (int n = 0; n < N; n++) r[n] = reduce_once(r[n]);
+#else
+    for (int n = 0; n < N; n++) r[n] = a[n] * b[n];
+#endif
+  }
+  // Two / three independent canonical products.
+  ZKP_HD static void mul2(const Fp& a0, const Fp& b0, const Fp& a1, const Fp& b1, Fp& r0, Fp& r1) {
+    mul_lazy2(a0, b0, a1, b1, r0, r1);
+    r0 = reduce_once(r0);
+    r1 = reduce_once(r1);
+  }
+  ZKP_HD static void mul3(const Fp& a0, const Fp& b0, const Fp& a1, const Fp& b1, const Fp& a2, const Fp& b2, Fp& r0,
+                          Fp& r1, Fp& r2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const Fp a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2};
+    Fp r[3];
+    mul_ps_raw_n<3>(a, b, r);
+    r0 = reduce_once(r[0]);
+    r1 = reduce_once(r[1]);
+    r2 = reduce_once(r[2]);
+#else
+    r0 = a0 * b0;
+    r1 = a1 * b1;
+    r2 = a2 * b2;
+#endif
+  }
 
   // Montgomery product a*b*R^-1 mod p.
   //
@@ -342,6 +387,66 @@ struct Fp {
     }
     t[7] = (u32)acc;
     return from_raw(t);
+  }
+
+  // N independent products r[n] = a[n] * b[n] * R^-1 in [0, 2p), computed in
+  // lockstep: the limb products of the N chains alternate, so each chain's
+  // dependent multiply-adds issue between the others' and a lone wave is not
+  // held by the latency of one chain.
+  template <int N>
+  __device__ __forceinline__ static void mul_ps_raw_n(const Fp (&A)[N], const Fp (&B)[N], Fp (&R)[N]) {
+    u32 m[N][8];
+    u32 t[N][8];
+    u64 acc[N];
+    u32 c2[N];
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+      acc[n] = 0;
+      c2[n] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int j = k - i;
+        if (j < 0 || j > 7) continue;
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+          if (first)
+            mac_first(acc[n], c2[n], A[n].v[i], B[n].v[j]);
+          else
+            mac(acc[n], c2[n], A[n].v[i], B[n].v[j]);
+        }
+        first = false;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int j = k - i;
+        if (i >= k || j < 1 || j > 7) continue;
+#pragma unroll
+        for (int n = 0; n < N; n++) mac(acc[n], c2[n], m[n][i], Params::P[j]);
+      }
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        if (k < 8) {
+          m[n][k] = (u32)acc[n] * Params::INV;
+        } else {
+          t[n][k - 8] = (u32)acc[n];
+        }
+      }
+      if (k < 8) {
+#pragma unroll
+        for (int n = 0; n < N; n++) mac(acc[n], c2[n], m[n][k], Params::P[0]);
+      }
+#pragma unroll
+      for (int n = 0; n < N; n++) acc[n] = (acc[n] >> 32) | ((u64)c2[n] << 32);
+    }
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+      t[n][7] = (u32)acc[n];
+      R[n] = from_raw(t[n]);
+    }
   }
 #endif
 

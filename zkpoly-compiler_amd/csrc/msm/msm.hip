@@ -21,9 +21,11 @@
 //     additions); runs that cross a chunk edge become (key, partial) pairs
 //     that the next level reduces the same way, so a bucket holding any share
 //     of the points (e.g. the scalar-1 bucket) costs log levels, not a serial walk.
-//  4. bucket reduction per window, S_w = sum_b b * B_b, in levels of running
-//     sums over segments of 2 (no serial walk over 2^(s-1) buckets; short
-//     dependent chains, since a chain of point additions is latency bound).
+//  4. bucket reduction per window, S_w = sum_b b * B_b, by halving levels
+//     (X, Y) -> (2 (X_2i + X_2i+1), X_2i+1 + Y_2i + Y_2i+1): X' and Y' run on
+//     different threads, two point operations deep per level, since a chain of
+//     point additions is latency bound; the last 256 items of a window are
+//     halved inside one workgroup (LDS), all levels in one launch.
 //  5. host: Horner over the window sums (s doublings each) and one inversion
 //     to affine -- the serial tail runs where a modular product is ~30x lower
 //     latency than one GPU wave (measured 738 ns per dependent product).
@@ -43,8 +45,7 @@ namespace msm {
 constexpr int THREADS = 256;
 constexpr uint32_t CHUNK = 32;   // sorted entries per thread at accumulation level 0
 constexpr uint32_t CHUNK_L = 8;  // (key, partial) entries per thread at the later levels
-constexpr uint32_t RED_G = 2;    // bucket-reduction segment (a power of two)
-constexpr uint32_t RED_G_LOG = 1;
+constexpr uint32_t RED_TAIL = 256;  // items per window finished inside one workgroup
 constexpr uint32_t SCALAR_BITS = 254;
 constexpr int MAX_LEVELS = 24;
 
@@ -610,31 +611,68 @@ __global__ void k_set_count(uint32_t* counts, const uint32_t* offsets, uint32_t 
 
 // ---------------------------------------------------------------- bucket reduction
 //
-// Per window: S = sum_j j X_j + sum_j Y_j over j < M.  A thread owns
-// j in [tG, tG + G) and returns Y'_t = sum (j - tG) X_j + sum Y_j and
-// X'_t = G * sum X_j, so that S = sum_t t X'_t + sum_t Y'_t (same form, M/G
-// items).  Level 0 has X = Y = buckets with bucket b at j = b - 1, which makes
-// S = sum_b b B_b.
+// Per window: S = sum_j j X_j + sum_j Y_j over j < M (M a power of two).  A
+// halving level maps the pair (2i, 2i + 1) to
+//   X'_i = 2 (X_2i + X_2i+1),   Y'_i = X_2i+1 + Y_2i + Y_2i+1,
+// which keeps S = sum_i i X'_i + sum_i Y'_i over M / 2 items; at M = 1,
+// S = Y_0.  Level 0 has X = Y = buckets, bucket b at j = b - 1, which makes
+// S = sum_b b B_b.  X' and Y' are computed by different threads (different
+// waves), so a level is two point operations deep.
+
+// One level over all windows: threads [0, Mo * n_windows) write X', the next
+// Mo * n_windows threads write Y' (Mo = M / 2).
 __global__ void __launch_bounds__(THREADS)
     k_reduce_level(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, uint32_t n_windows,
                    bool first, u32* __restrict__ Xo, u32* __restrict__ Yo) {
-  const uint32_t Mo = (M + RED_G - 1) / RED_G;
+  const uint32_t Mo = M >> 1;
+  const uint32_t items = Mo * n_windows;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= Mo * n_windows) return;
-  const uint32_t w = gid / Mo, t = gid % Mo;
-  const size_t base = (size_t)w * M;
-  const uint32_t lo = t * RED_G;
-  const uint32_t hi = lo + RED_G < M ? lo + RED_G : M;
-  G1XYZZ R = G1XYZZ::identity(), T = G1XYZZ::identity(), P = G1XYZZ::identity();
-  for (uint32_t j = hi; j-- > lo;) {
-    T = T.add(R);
-    R = R.add(G1XYZZ::load(X + (base + j) * 32));
-    if (!first) P = P.add(G1XYZZ::load(Y + (base + j) * 32));
+  if (gid >= 2 * items) return;
+  const bool yside = gid >= items;
+  const uint32_t it = yside ? gid - items : gid;
+  const size_t j0 = (size_t)(it / Mo) * M + 2 * (it % Mo);
+  const G1XYZZ x1 = G1XYZZ::load(X + (j0 + 1) * 32);
+  if (!yside) {
+    G1XYZZ::load(X + j0 * 32).add(x1).dbl().store(Xo + (size_t)it * 32);
+  } else {
+    const u32* src = first ? X : Y;
+    const G1XYZZ y1 = first ? x1 : G1XYZZ::load(Y + (j0 + 1) * 32);
+    G1XYZZ::load(src + j0 * 32).add(y1).add(x1).store(Yo + (size_t)it * 32);
   }
-  const G1XYZZ y = T.add(first ? R : P);
-  for (uint32_t k = 0; k < RED_G_LOG; k++) R = R.dbl();
-  R.store(Xo + ((size_t)w * Mo + t) * 32);
-  y.store(Yo + ((size_t)w * Mo + t) * 32);
+}
+
+// The last levels of window blockIdx.x (M <= RED_TAIL items) in LDS: threads
+// below THREADS / 2 compute X', the others Y'; out[w] = S_w.
+__global__ void __launch_bounds__(THREADS)
+    k_reduce_tail(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, bool first,
+                  u32* __restrict__ out) {
+  __shared__ G1XYZZ sx[RED_TAIL], sy[RED_TAIL];
+  static_assert(RED_TAIL / 2 <= THREADS / 2, "one pair per thread and side");
+  const size_t base = (size_t)blockIdx.x * M;
+  for (uint32_t j = threadIdx.x; j < M; j += THREADS) {
+    sx[j] = G1XYZZ::load(X + (base + j) * 32);
+    sy[j] = first ? sx[j] : G1XYZZ::load(Y + (base + j) * 32);
+  }
+  __syncthreads();
+  const bool yside = threadIdx.x >= THREADS / 2;
+  const uint32_t i = yside ? threadIdx.x - THREADS / 2 : threadIdx.x;
+  for (uint32_t P = M >> 1; P >= 1; P >>= 1) {
+    const bool act = i < P;
+    G1XYZZ r;
+    if (act) {
+      const G1XYZZ x1 = sx[2 * i + 1];
+      r = yside ? sy[2 * i].add(sy[2 * i + 1]).add(x1) : sx[2 * i].add(x1).dbl();
+    }
+    __syncthreads();
+    if (act) {
+      if (yside)
+        sy[i] = r;
+      else
+        sx[i] = r;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sy[0].store(out + (size_t)blockIdx.x * 32);
 }
 
 // ---------------------------------------------------------------- precompute
@@ -674,7 +712,7 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Device workspace for one MSM of `len` points (per table) and config cfg.
 struct Layout {
-  size_t hist, block_sums, cursor, keys0, vals0, counts, lvl_keys[2], lvl_pts[2], buckets, red[4], total;
+  size_t hist, block_sums, cursor, keys0, vals0, counts, lvl_keys[2], lvl_pts[2], buckets, red[4], sums, total;
   size_t matrix, keys_tmp, vals_tmp, slices;  // partition sort (when `part`)
   uint32_t max_slices;
   bool part;
@@ -686,7 +724,7 @@ struct Layout {
   uint32_t level_max[MAX_LEVELS];
   int levels;  // accumulation levels after level 0
   uint32_t red_m[MAX_LEVELS];
-  int red_levels;
+  int red_levels;  // one-level launches before the in-workgroup tail
 };
 
 // Partition-sort parameters for `len` points; false when the window is too wide
@@ -743,13 +781,13 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   L->levels = lv;
   if (max_even < 2) max_even = 2;
   if (max_odd < 2) max_odd = 2;
-  // reduction level sizes
+  // reduction level sizes (halving down to RED_TAIL items per window)
   uint32_t M = cfg.buckets;
   int rl = 0;
   uint64_t red_max0 = 0, red_max1 = 0;
-  while (M > 1) {
+  while (M > RED_TAIL) {
     if (rl >= MAX_LEVELS) return false;
-    M = (M + RED_G - 1) / RED_G;
+    M >>= 1;
     L->red_m[rl] = M;
     if (rl % 2 == 0) red_max0 = M > red_max0 ? M : red_max0;
     else red_max1 = M > red_max1 ? M : red_max1;
@@ -780,6 +818,7 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   L->red[1] = off; off = align256(off + (size_t)cfg.n_windows * red_max0 * 128);
   L->red[2] = off; off = align256(off + (size_t)cfg.n_windows * (red_max1 ? red_max1 : 1) * 128);
   L->red[3] = off; off = align256(off + (size_t)cfg.n_windows * (red_max1 ? red_max1 : 1) * 128);
+  L->sums = off; off = align256(off + (size_t)cfg.n_windows * 128);
   L->total = off;
   return true;
 }
@@ -921,22 +960,22 @@ static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, h
   const u32* X = reinterpret_cast<const u32*>(ws + L.buckets);
   const u32* Y = X;
   uint32_t M = cfg.buckets;
+  if (M < 2 || (M & (M - 1))) return hipErrorInvalidValue;  // 2^(s-1) buckets, s >= 2
   for (int rl = 0; rl < L.red_levels; rl++) {
     u32* Xo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2]);
     u32* Yo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2 + 1]);
     const uint32_t Mo = L.red_m[rl];
-    hipLaunchKernelGGL(k_reduce_level, dim3(div_ceil32((uint64_t)Mo * cfg.n_windows, THREADS)), dim3(THREADS), 0,
+    hipLaunchKernelGGL(k_reduce_level, dim3(div_ceil32(2ull * Mo * cfg.n_windows, THREADS)), dim3(THREADS), 0,
                        stream, X, Y, M, cfg.n_windows, rl == 0, Xo, Yo);
     ZKP_CHECK(hipGetLastError());
     X = Xo;
     Y = Yo;
     M = Mo;
   }
-  if (L.red_levels == 0) {
-    // one bucket per window (s = 1 is rejected earlier, so unreachable)
-    return hipErrorInvalidValue;
-  }
-  *out_dev = Y;
+  u32* sums = reinterpret_cast<u32*>(ws + L.sums);
+  hipLaunchKernelGGL(k_reduce_tail, dim3(cfg.n_windows), dim3(THREADS), 0, stream, X, Y, M, L.red_levels == 0, sums);
+  ZKP_CHECK(hipGetLastError());
+  *out_dev = sums;
   pt.mark("reduce");
   return hipSuccess;
 }

@@ -393,14 +393,13 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         if (LAZY) {
           if (!skip1) {
             const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
-            t1 = Fr::mul_lazy(t1, w1);
-            t3 = Fr::mul_lazy(t3, w1);
+            Fr::mul_lazy2(x[u][1], w1, x[u][3], w1, t1, t3);
           }
           const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
           const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
           const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
-          const Fr t2 = Fr::mul_lazy(a2, w2);
-          const Fr t4 = Fr::mul_lazy(a3, w3);
+          Fr t2, t4;
+          Fr::mul_lazy2(a2, w2, a3, w3, t2, t4);
           const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
           x[u][0] = Fr::add_raw(a0n, t2);
           x[u][2] = Fr::sub_2p(a0n, t2);
@@ -636,54 +635,55 @@ __global__ void __launch_bounds__(THREADS)
 
 // ---------------------------------------------------------------- radix-4 pass
 //
-// Same tile geometry as dit_tile, but stages are taken two at a time: a thread
-// owns quads of rows {k0, k0+h, k0+2h, k0+3h} (h = 2^m, bits m and m+1 of k0
-// clear), which stages m and m+1 connect only among themselves.  Stage m uses
-// one twiddle for both of its butterflies (k0 and k0+2h agree modulo 2^(s0+m)),
-// stage m+1 two.  Per LDS round trip a thread does 2 quads = 8 butterflies
-// with 4 independent products per stage; an odd last stage runs radix-2.
+// Same tile geometry as dit_tile (2^d rows x 2^c_log columns); the stages run
+// two at a time (pass_stages).
 __device__ __forceinline__ uint64_t tile_index(uint64_t base, uint64_t cstride, uint64_t kstride, uint32_t c,
                                                uint32_t k) {
   return base + c * cstride + (uint64_t)k * kstride;
 }
 
-// Butterflies are lazy (Harvey): values stay in [0, 4p) between stages and
-// passes; the transform's last pass (final) stores canonical values.
-__device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
-                                            const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
-                                            uint32_t c_log, uint64_t tile, bool final_pass) {
-  const uint32_t C = 1u << c_log;
-  const uint32_t R = 1u << d;
-  const uint32_t T = R << c_log;
+// Global geometry of one pass tile (tile_index(base, cstride, kstride, c, k)).
+struct TileGeom {
   uint64_t base, cstride, kstride;
-  if (s0 == 0) {
-    base = (tile << c_log) << d;
-    cstride = R;
-    kstride = 1;
-  } else {
-    const uint64_t lo_blocks = (1ull << s0) >> c_log;
-    const uint64_t hi = tile / lo_blocks;
-    const uint64_t lo_block = tile % lo_blocks;
-    base = lo_block * C + (hi << (s0 + d));
-    cstride = 1;
-    kstride = 1ull << s0;
-  }
-  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
-    uint32_t c, k;
-    if (cstride == 1) {
-      c = e & (C - 1);
-      k = e >> c_log;
-    } else {
-      k = e & (R - 1);
-      c = e >> d;
-    }
-    const uint4* src = reinterpret_cast<const uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
-    const uint32_t pos = (k << c_log) | c;
-    lds0[pos] = src[0];
-    lds1[pos] = src[1];
-  }
-  __syncthreads();
+};
 
+__device__ __forceinline__ TileGeom tile_geom(uint32_t s0, uint32_t d, uint32_t c_log, uint64_t tile) {
+  if (s0 == 0) return TileGeom{(tile << c_log) << d, 1ull << d, 1};
+  // 2^(s0 - c_log) column blocks per row block (c_log <= s0)
+  const uint32_t lo_bits = s0 - c_log;
+  const uint64_t hi = tile >> lo_bits;
+  const uint64_t lo_block = tile & ((1ull << lo_bits) - 1);
+  return TileGeom{lo_block << c_log | (hi << (s0 + d)), 1, 1ull << s0};
+}
+
+// Element e of a tile: its LDS position (row k, column c -> k * C + c) and its
+// global index.  Consecutive e walk the contiguous direction in memory.
+__device__ __forceinline__ uint64_t tile_elem(const TileGeom& g, uint32_t d, uint32_t c_log, uint32_t e,
+                                              uint32_t& pos) {
+  uint32_t c, k;
+  if (g.cstride == 1) {
+    c = e & ((1u << c_log) - 1);
+    k = e >> c_log;
+  } else {
+    k = e & ((1u << d) - 1);
+    c = e >> d;
+  }
+  pos = (k << c_log) | c;
+  return tile_index(g.base, g.cstride, g.kstride, c, k);
+}
+
+// Stages [s0, s0 + d) on the tile in LDS; ends with a barrier.  Butterflies are
+// lazy (Harvey): values stay in [0, 4p).  Stages go two at a time: a thread
+// owns quads of rows {k0, k0+h, k0+2h, k0+3h} (h = 2^m, bits m and m+1 of k0
+// clear), which stages m and m+1 connect only among themselves; stage m uses
+// one twiddle for both of its butterflies (k0 and k0+2h agree modulo 2^(s0+m)),
+// stage m+1 two.  Per LDS round trip a thread does 2 quads = 8 butterflies with
+// 4 independent products per stage; an odd last stage runs radix-2.
+__device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __restrict__ lds1,
+                                            const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+                                            uint32_t c_log, const TileGeom& geo) {
+  const uint32_t C = 1u << c_log;
+  const uint32_t T = (1u << d) << c_log;
   uint32_t m = 0;
   for (; m + 1 < d; m += 2) {
     const uint32_t s = s0 + m;
@@ -703,8 +703,8 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
         const uint32_t c = gg & (C - 1);
         const uint32_t qr = gg >> c_log;
         const uint32_t k0 = ((qr >> m) << (m + 2)) | (qr & (h - 1));
-        const uint64_t i0 = tile_index(base, cstride, kstride, c, k0);
-        const uint64_t i1 = tile_index(base, cstride, kstride, c, k0 + h);
+        const uint64_t i0 = tile_index(geo.base, geo.cstride, geo.kstride, c, k0);
+        const uint64_t i1 = tile_index(geo.base, geo.cstride, geo.kstride, c, k0 + h);
         w1[u] = Fr::load(tw + (((i0 & mask1) << sh1) << 3));
         w2[u] = Fr::load(tw + (((i0 & mask2) << sh2) << 3));
         w3[u] = Fr::load(tw + (((i1 & mask2) << sh2) << 3));
@@ -716,13 +716,12 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
       }
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        const Fr t1 = Fr::mul_lazy(x[u][1], w1[u]);
-        const Fr t3 = Fr::mul_lazy(x[u][3], w1[u]);
+        Fr t1, t3, t2, t4;
+        Fr::mul_lazy2(x[u][1], w1[u], x[u][3], w1[u], t1, t3);
         const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
         const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
         const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
-        const Fr t2 = Fr::mul_lazy(a2, w2[u]);
-        const Fr t4 = Fr::mul_lazy(a3, w3[u]);
+        Fr::mul_lazy2(a2, w2[u], a3, w3[u], t2, t4);
         const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
         x[u][0] = Fr::add_raw(a0n, t2);
         x[u][2] = Fr::sub_2p(a0n, t2);
@@ -749,7 +748,7 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
       const uint32_t ka = ((q >> m) << (m + 1)) | (q & (half - 1));
       const uint32_t pa = (ka << c_log) | c;
       const uint32_t pb = ((ka + half) << c_log) | c;
-      const Fr w = Fr::load(tw + (((tile_index(base, cstride, kstride, c, ka) & jmask) << tw_shift) << 3));
+      const Fr w = Fr::load(tw + (((tile_index(geo.base, geo.cstride, geo.kstride, c, ka) & jmask) << tw_shift) << 3));
       const Fr x = Fr::reduce_2p(lds_get(lds0, lds1, pa));
       const Fr t = Fr::mul_lazy(lds_get(lds0, lds1, pb), w);
       lds_put(lds0, lds1, pa, Fr::add_raw(x, t));
@@ -757,18 +756,17 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
     }
     __syncthreads();
   }
+}
 
+// The tile back to memory; the transform's last pass (final) stores
+// canonical values.
+__device__ __forceinline__ void pass_store(const uint4* __restrict__ lds0, const uint4* __restrict__ lds1,
+                                           const PolyView& v, const TileGeom& geo, uint32_t d, uint32_t c_log,
+                                           bool final_pass) {
+  const uint32_t T = 1u << (d + c_log);
   for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
-    uint32_t c, k;
-    if (cstride == 1) {
-      c = e & (C - 1);
-      k = e >> c_log;
-    } else {
-      k = e & (R - 1);
-      c = e >> d;
-    }
-    uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
-    const uint32_t pos = (k << c_log) | c;
+    uint32_t pos;
+    uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_elem(geo, d, c_log, e, pos)));
     if (final_pass) {
       const Fr y = Fr::reduce_4p(lds_get(lds0, lds1, pos));
       dst[0] = make_uint4(y.v[0], y.v[1], y.v[2], y.v[3]);
@@ -778,6 +776,23 @@ __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __r
       dst[1] = lds1[pos];
     }
   }
+}
+
+// One tile per workgroup: load, stages, store.
+__device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
+                                            const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+                                            uint32_t c_log, uint64_t tile, bool final_pass) {
+  const uint32_t T = 1u << (d + c_log);
+  const TileGeom geo = tile_geom(s0, d, c_log, tile);
+  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    uint32_t pos;
+    const uint4* src = reinterpret_cast<const uint4*>(v.elem(tile_elem(geo, d, c_log, e, pos)));
+    lds0[pos] = src[0];
+    lds1[pos] = src[1];
+  }
+  __syncthreads();
+  pass_stages(lds0, lds1, tw, log_n, s0, d, c_log, geo);
+  pass_store(lds0, lds1, v, geo, d, c_log, final_pass);
 }
 
 __global__ void __launch_bounds__(THREADS)
@@ -1154,9 +1169,9 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
     }
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
     for (int p = 0; p < plan.np; p++) {
-      hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)plan.pass[p].tiles), dim3(THREADS), 0, stream, v, twiddle,
-                         (uint32_t)log_len, plan.pass[p].s0, plan.pass[p].d, plan.pass[p].c_log,
-                         (uint32_t)(p + 1 == plan.np));
+      const PassPlan& pp = plan.pass[p];
+      hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)pp.tiles), dim3(THREADS), 0, stream, v, twiddle,
+                         (uint32_t)log_len, pp.s0, pp.d, pp.c_log, (uint32_t)(p + 1 == plan.np));
       if ((err = hipGetLastError()) != hipSuccess || (err = mark(1)) != hipSuccess) return err;
     }
   }
