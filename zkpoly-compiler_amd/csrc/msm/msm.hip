@@ -469,18 +469,27 @@ __global__ void __launch_bounds__(THREADS)
 }
 
 // Every slice takes its run of each bucket from the cursor (one atomic per
-// nonzero bucket; a one-slice segment owns the whole bucket) and ranks its
-// entries in LDS.  The order inside a bucket is not kept.
+// nonzero bucket; a one-slice segment owns the whole bucket), then moves its
+// entries in rounds of SEG_ROUND: a round is ranked by bucket in LDS, staged in
+// bucket order and written out so that each bucket's entries of the round land
+// as one contiguous run (coalesced stores instead of one scattered store per
+// entry).  The order inside a bucket is not kept.
+constexpr uint32_t SEG_ROUND = 8192;  // entries staged in LDS per round (64 KB)
+
 __global__ void __launch_bounds__(THREADS)
     k_seg_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ slice_prefix, SortPlan sp,
                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
-  extern __shared__ uint32_t h[];  // 2 * nb: counts, then bases
+  extern __shared__ uint32_t h[];  // cnt, lbase, gcur (nb each), then the staged keys and values
+  __shared__ uint32_t sh[THREADS];
   SegSlice sl;
-  if (!seg_slice(offsets, slice_prefix, sp, &sl)) return;
+  if (!seg_slice(offsets, slice_prefix, sp, &sl)) return;  // uniform per block
   const uint32_t nb = 1u << sp.low_bits, mask = nb - 1;
   uint32_t* cnt = h;
-  uint32_t* base = h + nb;
+  uint32_t* lbase = h + nb;
+  uint32_t* gcur = h + 2 * nb;
+  uint32_t* skey = h + 3 * nb;
+  uint32_t* sval = skey + SEG_ROUND;
   for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = 0;
   __syncthreads();
   for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) atomicAdd(&cnt[keys_in[e] & mask], 1u);
@@ -488,17 +497,41 @@ __global__ void __launch_bounds__(THREADS)
   uint32_t* cur = cursor + (uint64_t)sl.cb * nb;
   for (uint32_t b = threadIdx.x; b < nb; b += THREADS) {
     const uint32_t c = cnt[b];
-    base[b] = sl.single ? cur[b] : (c ? atomicAdd(&cur[b], c) : 0u);
-    cnt[b] = 0;
+    gcur[b] = sl.single ? cur[b] : (c ? atomicAdd(&cur[b], c) : 0u);
   }
-  __syncthreads();
-  for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) {
-    const uint32_t key = keys_in[e];
-    const uint32_t pos = base[key & mask] + atomicAdd(&cnt[key & mask], 1u);
-    keys_out[pos] = key;
-    vals_out[pos] = vals_in[e];
+  for (uint32_t r0 = sl.lo; r0 < sl.hi; r0 += SEG_ROUND) {
+    const uint32_t r1 = r0 + SEG_ROUND < sl.hi ? r0 + SEG_ROUND : sl.hi;
+    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = 0;
+    __syncthreads();
+    for (uint32_t e = r0 + threadIdx.x; e < r1; e += THREADS) atomicAdd(&cnt[keys_in[e] & mask], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) lbase[b] = cnt[b];
+    __syncthreads();
+    lds_exclusive_scan(lbase, nb, sh);
+    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = lbase[b];  // staging cursor
+    __syncthreads();
+    for (uint32_t e = r0 + threadIdx.x; e < r1; e += THREADS) {
+      const uint32_t key = keys_in[e];
+      const uint32_t slot = atomicAdd(&cnt[key & mask], 1u);
+      skey[slot] = key;
+      sval[slot] = vals_in[e];
+    }
+    __syncthreads();
+    for (uint32_t slot = threadIdx.x; slot < r1 - r0; slot += THREADS) {
+      const uint32_t key = skey[slot];
+      const uint32_t b = key & mask;
+      const uint32_t dst = gcur[b] + (slot - lbase[b]);
+      keys_out[dst] = key;
+      vals_out[dst] = sval[slot];
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) gcur[b] += cnt[b] - lbase[b];  // this round's entries
+    __syncthreads();
   }
 }
+
+// Dynamic LDS of k_seg_scatter for 2^low_bits buckets per segment.
+static inline uint32_t seg_scatter_lds(uint32_t low_bits) { return (3u * (1u << low_bits) + 2u * SEG_ROUND) * 4u; }
 
 // ---------------------------------------------------------------- accumulation
 
@@ -908,8 +941,12 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(L.H, 4096)), dim3(THREADS), 0, stream, cursor, (uint64_t)L.H);
     hipLaunchKernelGGL(k_seg_hist, dim3(L.max_slices), dim3(THREADS), nbk * 4, stream, ktmp, matrix, slices, sp, cursor);
     hipLaunchKernelGGL(k_seg_scan, dim3(sp.nc), dim3(THREADS), nbk * 4, stream, matrix, sp, cursor);
-    hipLaunchKernelGGL(k_seg_scatter, dim3(L.max_slices), dim3(THREADS), nbk * 8, stream, ktmp, vtmp, matrix, slices, sp,
-                       cursor, keys0, vals0);
+    const uint32_t lds_seg = seg_scatter_lds(sp.low_bits);
+    if (lds_seg > 64 * 1024)
+      ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_seg_scatter),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_seg));
+    hipLaunchKernelGGL(k_seg_scatter, dim3(L.max_slices), dim3(THREADS), lds_seg, stream, ktmp, vtmp, matrix, slices,
+                       sp, cursor, keys0, vals0);
     ZKP_CHECK(hipGetLastError());
     pt.mark("segsort");
   } else {
