@@ -144,6 +144,27 @@ def test_reference_abi_parts_and_precompute(libs, stream):
         assert list(res.data) == list(want), (window, target, parts)
 
 
+def test_parts_with_a_heavy_bucket(libs, stream):
+    """Window 16 in two parts with 30 % of the scalars equal to 1: the first
+    part stores its buckets and the second adds to them, while the scalar-1
+    bucket is a run far longer than the direct run pass closes, so the
+    chunked levels finish it in both parts."""
+    n = 1 << 14
+    ks = random_elements(n, 8100)
+    ss = random_elements(n, 8101)
+    rng = np.random.Generator(np.random.PCG64(8102))
+    ss[rng.permutation(n)[: (3 * n) // 10]] = mont_limbs(1)
+    pb = gpu_bases(libs, stream, ks)
+    t0 = np.ascontiguousarray(pb.download(np.uint64, n * 8).reshape(n, 8))
+    cfg = MsmConfig(window_size=16, target_window=16, parts=2)
+    op = MSM(libs, cfg)
+    size = op.get_buffer_size(n)[0]
+    buf = hip.DeviceBuffer(size)
+    res = Point()
+    op.get_fn().f([GpuBuffer(buf.addr, size), res], [PointArray(t0.ctypes.data, n), ScalarArray(n, ss.ctypes.data, "cpu")])
+    assert list(res.data) == list(closed_form(ks, ss))
+
+
 @pytest.mark.parametrize("variant", ["uniform", "zeros_ones"])
 def test_workload_2_20(libs, stream, variant):
     """Config 3: 2^20 points, window 16, closed-form check; the variant has 10 % zeros, 10 % ones."""

@@ -619,11 +619,47 @@ __global__ void __launch_bounds__(THREADS)
   accumulate_chunk<true, CHUNK>(keys, vals, points, counts, 0, fresh, buckets, next_keys, next_pts);
 }
 
-// Level >= 1: entries are (key, XYZZ partial).
+// Level >= 1: entries are (key, XYZZ partial).  With run_flag given, the
+// level runs only when *run_flag is set (k_close_runs found a long run).
 __global__ void __launch_bounds__(THREADS)
     k_accumulate(const uint32_t* __restrict__ keys, const u32* __restrict__ pts, uint32_t* counts, uint32_t level,
-                 bool fresh, u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
+                 bool fresh, u32* buckets, uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts,
+                 const uint32_t* __restrict__ run_flag) {
+  if (run_flag && *run_flag == 0u) return;
   accumulate_chunk<false, CHUNK_L>(keys, nullptr, pts, counts, level, fresh, buckets, next_keys, next_pts);
+}
+
+// After the first chunked level the open pieces of a bucket form short runs
+// of equal keys (for scalars spread over the buckets, a bucket that crosses
+// chunk edges leaves two to four slots), which fixed chunk edges would keep
+// deferring level after level.  Here the thread at the first slot of every
+// run of at most RUN_MAX slots adds the run, closes the bucket and marks the
+// slots consumed (ZZ = 0, the identity), so the bucket is completed once.  A
+// longer run (a bucket holding a large share of the points) raises *run_flag
+// and stays for the chunked levels, which see the consumed slots as identity.
+constexpr uint32_t RUN_MAX = 16;
+
+__global__ void __launch_bounds__(THREADS)
+    k_close_runs(const uint32_t* __restrict__ keys, u32* __restrict__ pts, const uint32_t* __restrict__ counts,
+                 uint32_t level, bool fresh, u32* buckets, uint32_t* __restrict__ run_flag) {
+  const uint32_t N = counts[level];
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const uint32_t key = keys[s];
+  if (s > 0 && keys[s - 1] == key) return;  // not the first slot of its run
+  uint32_t e = s + 1;
+  while (e < N && e - s <= RUN_MAX && keys[e] == key) e++;
+  if (e - s > RUN_MAX) {
+    *run_flag = 1u;
+    return;
+  }
+  G1XYZZ acc = G1XYZZ::identity();
+  for (uint32_t i = s; i < e; i++) {
+    u32* p = pts + (size_t)i * 32;
+    acc = acc.add(G1XYZZ::load(p));
+    Fq::zero().store(p + 16);
+  }
+  bucket_close(buckets, key, acc, fresh);
 }
 
 __global__ void __launch_bounds__(THREADS) k_zero_u32(uint32_t* p, uint64_t n) {
@@ -976,6 +1012,13 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
                      d_points, counts, fresh, buckets, nk, np);
   ZKP_CHECK(hipGetLastError());
   pt.mark("accumulate0");
+  // level 1 chunked, then the short runs of level 2 closed directly; the later
+  // chunked levels only run when a long run remains.  A top window of few
+  // bits puts its digits in a handful of buckets (runs far above RUN_MAX), so
+  // then every level runs and the run pass is left out.
+  uint32_t* run_flag = counts + MAX_LEVELS;
+  const uint32_t top_bits = SCALAR_BITS - (cfg.actual_windows - 1) * cfg.s;
+  const bool close_runs = top_bits >= 10;
   for (int lv = 0; lv < L.levels; lv++) {
     const int cur = lv % 2, nxt = 1 - cur;
     const bool last = lv == L.levels - 1;
@@ -984,8 +1027,16 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
                        reinterpret_cast<const uint32_t*>(ws + L.lvl_keys[cur]),
                        reinterpret_cast<const u32*>(ws + L.lvl_pts[cur]), counts, (uint32_t)(lv + 1), fresh, buckets,
                        last ? nullptr : reinterpret_cast<uint32_t*>(ws + L.lvl_keys[nxt]),
-                       last ? nullptr : reinterpret_cast<u32*>(ws + L.lvl_pts[nxt]));
+                       last ? nullptr : reinterpret_cast<u32*>(ws + L.lvl_pts[nxt]),
+                       lv == 0 || !close_runs ? nullptr : run_flag);
     ZKP_CHECK(hipGetLastError());
+    if (lv == 0 && !last && close_runs) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(THREADS), 0, stream, run_flag, (uint64_t)1);
+      hipLaunchKernelGGL(k_close_runs, dim3(grid_for(L.level_max[1], 1u << 30)), dim3(THREADS), 0, stream,
+                         reinterpret_cast<const uint32_t*>(ws + L.lvl_keys[nxt]),
+                         reinterpret_cast<u32*>(ws + L.lvl_pts[nxt]), counts, 2u, fresh, buckets, run_flag);
+      ZKP_CHECK(hipGetLastError());
+    }
   }
   pt.mark("levels");
   return hipSuccess;
