@@ -464,7 +464,8 @@ def main():
             "frac": round(int_rate / INT_MUL_PEAK, 4),
         },
         "bitrev_head_avg_ms": round(sum(per_kind[0]) / max(1, len(per_kind[0])), 5),
-        "valu_rocprof": valu_from_profiles(["dit_pass_r4", "bitrev_head", "k_accumulate0", "k_reduce_level"]),
+        "valu_rocprof": valu_from_profiles(["dit_pass_r4", "bitrev_head_stream", "k_accumulate0", "k_close_runs",
+                                            "k_reduce_level", "k_reduce_tail", "k_seg_scatter"]),
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

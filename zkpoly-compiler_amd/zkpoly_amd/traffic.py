@@ -14,6 +14,11 @@ traffic is well above its algorithmic bytes re-reads data.
 rocprofv3's derived VALUBusy (percent of time the vector ALUs issue) and
 VALUUtilization (percent of active lanes) per kernel, from runs of
 `rocprofv3 --pmc VALUBusy VALUUtilization` over bench.py's NTT and MSM legs.
+
+`python -m zkpoly_amd.traffic trace DIR TITLE COMMAND OUT.md [KERNEL N]` writes
+the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (DIR holds its
+run_kernel_stats.csv) as markdown, with the command that made it, and the
+average of KERNEL's last N dispatches (bench.py's timed region).
 """
 from __future__ import annotations
 
@@ -137,6 +142,31 @@ def summarize_valu(ntt_csv: str, msm_csv: str, out_json: str, out_md: str) -> di
     return out
 
 
+def summarize_trace(trace_dir: str, title: str, command: str, out_md: str, tail_kernel: str = "",
+                    tail_n: int = 0) -> None:
+    with open(f"{trace_dir}/run_kernel_stats.csv") as f:
+        stats = list(csv.DictReader(f))
+    lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
+             "## Kernel stats (all dispatches of the run)", "",
+             "| kernel | calls | avg ms | min ms | max ms | % time |", "|---|---|---|---|---|---|"]
+    for r in stats:
+        name = r["Name"].split("(")[0]
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | {float(r['MinNs']) / 1e6:.4f} | "
+                     f"{float(r['MaxNs']) / 1e6:.4f} | {float(r['Percentage']):.1f} |")
+    if tail_kernel and tail_n:
+        # the run's last tail_n dispatches of one kernel: bench.py's timed region, after its autotuning
+        with open(f"{trace_dir}/run_kernel_trace.csv") as f:
+            durs = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in csv.DictReader(f)
+                    if r["Kernel_Name"].split("(")[0].split("::")[-1] == tail_kernel]
+        last = durs[-tail_n:]
+        if last:
+            lines += ["", f"Timed region (the last {len(last)} `{tail_kernel}` dispatches, after the autotuner's "
+                      f"trial launches): {sum(last) / len(last):.4f} ms average, "
+                      f"{min(last):.4f}-{max(last):.4f} ms."]
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         run()
@@ -144,6 +174,8 @@ if __name__ == "__main__":
         print(json.dumps(summarize(*sys.argv[2:6]), indent=1))
     elif len(sys.argv) >= 6 and sys.argv[1] == "valu":
         print(json.dumps(summarize_valu(*sys.argv[2:6]), indent=1))
+    elif len(sys.argv) >= 6 and sys.argv[1] == "trace":
+        summarize_trace(*sys.argv[2:6], *(sys.argv[6:7] + [int(a) for a in sys.argv[7:8]]))
     else:
         print(__doc__)
         sys.exit(2)
