@@ -70,7 +70,7 @@ def test_ssip_ntt_random_omega_rotated_slice(libs, stream, k):
 DEFAULT_MIN_COL_LOG = 2
 
 
-@pytest.mark.parametrize("variant,min_col", [(0, 1), (0, 3), (0, 6), (1, 2), (2, 2)])
+@pytest.mark.parametrize("variant,min_col", [(0, 1), (0, 3), (0, 6), (1, 2), (2, 2), (3, 1), (3, 3)])
 def test_variants_and_tunings(libs, stream, variant, min_col):
     """Every kernel plan gives the same best_fft network: the legacy radix-2
     plan and the fused-head radix-4 plan at several pass depths."""
@@ -101,7 +101,7 @@ def test_head_variants_at_large_sizes(libs, stream, k):
     data = random_elements(n, 4100 + k)
     w = random_elements(1, 4200 + k)[0]
     want = orc.best_fft(data, w, k, THREADS)
-    for variant in (0, 2, 1):
+    for variant in (0, 2, 1, 3):
         old = set_variant(libs, variant)
         try:
             got, _ = gpu_ntt(libs, stream, data, w, rotate=(n >> 3) + 7, sliced=True)

@@ -292,7 +292,6 @@ __device__ __forceinline__ void lds_put(uint4* l0, uint4* l1, uint32_t p, const 
 }
 
 constexpr uint32_t HEAD_BUF = HEAD_W * HEAD_PITCH;  // one 32 x 33 tile plane
-constexpr int HEAD_LOADS = HEAD_W * HEAD_W / THREADS;
 
 // Two tile buffers per plane (x at M at offset 0, x at rev M at HEAD_BUF) and
 // the head's twiddles r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1).
@@ -301,21 +300,22 @@ struct HeadLds {
   uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
 };
 
-// Per-thread staging of the head's global loads: element e = j * THREADS +
-// threadIdx.x of the tile at M (a) and of the tile at rev M (b), j < 4.
+// Per-thread staging of the head's global loads: element e = j * NT +
+// threadIdx.x of the tile at M (a) and of the tile at rev M (b), j < 1024 / NT.
 struct HeadStaged {
   uint4 a0, a1, b0, b1;
 };
-static_assert(HEAD_LOADS == 4, "head staging is written for four loads per thread");
 
-// Load j of the thread (element e = j * THREADS + threadIdx.x of the tile at M
-// and of the tile at rev M).  The loads are unconditional: the index is
-// clamped for t < 5, and x at M is read twice when M == rev M.  Callers keep
-// the four results in named values so the staging stays in registers.
+// Load j of the thread (element e = j * NT + threadIdx.x of the tile at M and
+// of the tile at rev M) for a workgroup of NT threads.  The loads are
+// unconditional: the index is clamped for t < 5, and x at M is read twice when
+// M == rev M.  Callers keep the results in named values so the staging stays
+// in registers.
+template <int NT>
 __device__ __forceinline__ HeadStaged head_fetch(const PolyView& v, uint32_t t, uint32_t shift_a, uint64_t M,
                                                  uint64_t Mr, uint32_t j) {
   const uint32_t W = 1u << t;
-  const uint32_t e = (threadIdx.x + j * THREADS) & (W * W - 1);
+  const uint32_t e = (threadIdx.x + j * NT) & (W * W - 1);
   const uint32_t a = e >> t, b = e & (W - 1);
   const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
   const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
@@ -323,10 +323,11 @@ __device__ __forceinline__ HeadStaged head_fetch(const PolyView& v, uint32_t t, 
 }
 
 // Staged load j into the two LDS tile buffers (x at M at offset 0, x at rev M
-// at HEAD_BUF); the caller puts a barrier after the four.
+// at HEAD_BUF); the caller puts a barrier after the last.
+template <int NT>
 __device__ __forceinline__ void head_place(HeadLds& L, uint32_t t, bool pair, uint32_t j, const HeadStaged& g) {
   const uint32_t W = 1u << t;
-  const uint32_t e = threadIdx.x + j * THREADS;
+  const uint32_t e = threadIdx.x + j * NT;
   if (e < W * W) {
     const uint32_t p = (e >> t) * HEAD_PITCH + (e & (W - 1));
     L.p0[p] = g.a0;
@@ -342,8 +343,10 @@ __device__ __forceinline__ void head_place(HeadLds& L, uint32_t t, bool pair, ui
 // ends with a barrier.  LAZY keeps values in [0, 4p) (Harvey butterflies, as in
 // the passes), for callers whose later passes accept such inputs; otherwise
 // every stage output is canonical.
-template <bool LAZY>
+template <bool LAZY, int NT>
 __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
+  constexpr int U = 512 / NT;   // quads per thread per round (radix-4 stages)
+  constexpr int R2 = 1024 / NT;  // butterflies per thread per round (radix-2 stage)
   const uint32_t W = 1u << t;
   // tile 0 = y at M (in the buffer of x at Mr), tile 1 = y at Mr (buffer of x at M)
   const uint32_t y0 = pair ? HEAD_BUF : 0;
@@ -362,13 +365,13 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
     const uint32_t per_tile_q = W * (W >> 2);
     const uint32_t totq = (pair ? 2 : 1) * per_tile_q;
     const bool skip1 = m == 0 && unit0;
-    for (uint32_t u0 = threadIdx.x; u0 < totq; u0 += 2 * THREADS) {
-      Fr x[2][4];
-      uint32_t pos[2][4], jw[2];
-      bool live[2];
+    for (uint32_t u0 = threadIdx.x; u0 < totq; u0 += U * NT) {
+      Fr x[U][4];
+      uint32_t pos[U][4], jw[U];
+      bool live[U];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const uint32_t uq = u0 + u * THREADS;
+      for (int u = 0; u < U; u++) {
+        const uint32_t uq = u0 + u * NT;
         live[u] = uq < totq;
         const uint32_t uu = live[u] ? uq : u0;
         const uint32_t tile = uu >= per_tile_q ? 1 : 0;
@@ -386,7 +389,7 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
+      for (int u = 0; u < U; u++) {
         Fr t1 = x[u][1], t3 = x[u][3];
         const Fr w2 = lds_get(L.r0, L.r1, jw[u] << (t - 2 - m));
         const Fr w3 = lds_get(L.r0, L.r1, (jw[u] + h) << (t - 2 - m));
@@ -422,7 +425,7 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
+      for (int u = 0; u < U; u++) {
         if (!live[u]) continue;
 #pragma unroll
         for (int r4 = 0; r4 < 4; r4++) lds_put(L.p0, L.p1, pos[u][r4], x[u][r4]);
@@ -433,13 +436,13 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
   if (m < t) {  // odd last stage, radix-2
     const uint32_t half = 1u << m;
     const bool skip = m == 0 && unit0;
-    for (uint32_t u0 = threadIdx.x; u0 < total; u0 += 4 * THREADS) {
-      Fr x[4], y[4];
-      uint32_t pa[4], pb[4], wi[4];
-      bool live[4];
+    for (uint32_t u0 = threadIdx.x; u0 < total; u0 += R2 * NT) {
+      Fr x[R2], y[R2];
+      uint32_t pa[R2], pb[R2], wi[R2];
+      bool live[R2];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t u = u0 + j * THREADS;
+      for (int j = 0; j < R2; j++) {
+        const uint32_t u = u0 + j * NT;
         live[j] = u < total;
         const uint32_t uu = live[j] ? u : u0;
         const uint32_t tile = uu >= per_tile ? 1 : 0;
@@ -457,13 +460,13 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
       }
       if (!skip) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < R2; j++) {
           const Fr w = lds_get(L.r0, L.r1, wi[j]);
           y[j] = LAZY ? Fr::mul_lazy(y[j], w) : y[j] * w;
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < R2; j++) {
         const Fr tt = y[j];
         if (LAZY) {
           const Fr xr = Fr::reduce_2p(x[j]);
@@ -475,7 +478,7 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < R2; j++) {
         if (!live[j]) continue;
         lds_put(L.p0, L.p1, pa[j], x[j]);
         lds_put(L.p0, L.p1, pb[j], y[j]);
@@ -486,11 +489,12 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
 }
 
 // Both y tiles back in place (y at M from the buffer at y0, y at Mr from 0).
+template <int NT>
 __device__ __forceinline__ void head_store(const HeadLds& L, const PolyView& v, uint32_t t, uint32_t shift_a,
                                            uint64_t M, uint64_t Mr, bool pair) {
   const uint32_t W = 1u << t;
   const uint32_t y0 = pair ? HEAD_BUF : 0;
-  for (uint32_t e = threadIdx.x; e < W * W; e += THREADS) {
+  for (uint32_t e = threadIdx.x; e < W * W; e += NT) {
     const uint32_t a = e >> t, b = e & (W - 1);
     const uint32_t p = rev_bits(b, t) * HEAD_PITCH + rev_bits(a, t);
     uint4* d = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
@@ -506,6 +510,7 @@ __device__ __forceinline__ void head_store(const HeadLds& L, const PolyView& v, 
 
 // One tile pair {M, rev M} per workgroup.  The caller has written L.r
 // (visible after the first barrier below).
+// (256 threads.)
 template <bool LAZY>
 __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint64_t M) {
   const uint32_t mbits = log_n - 2 * t;
@@ -513,15 +518,15 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
   if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
   const bool pair = M != Mr;
   const uint32_t shift_a = log_n - t;
-  const HeadStaged g0 = head_fetch(v, t, shift_a, M, Mr, 0), g1 = head_fetch(v, t, shift_a, M, Mr, 1),
-                   g2 = head_fetch(v, t, shift_a, M, Mr, 2), g3 = head_fetch(v, t, shift_a, M, Mr, 3);
-  head_place(L, t, pair, 0, g0);
-  head_place(L, t, pair, 1, g1);
-  head_place(L, t, pair, 2, g2);
-  head_place(L, t, pair, 3, g3);
+  const HeadStaged g0 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 0), g1 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 1),
+                   g2 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 2), g3 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 3);
+  head_place<THREADS>(L, t, pair, 0, g0);
+  head_place<THREADS>(L, t, pair, 1, g1);
+  head_place<THREADS>(L, t, pair, 2, g2);
+  head_place<THREADS>(L, t, pair, 3, g3);
   __syncthreads();
-  head_stages<LAZY>(L, t, pair);
-  head_store(L, v, t, shift_a, M, Mr, pair);
+  head_stages<LAZY, THREADS>(L, t, pair);
+  head_store<THREADS>(L, v, t, shift_a, M, Mr, pair);
 }
 
 // L.r[i] = tw[i << (log_n - t)] for i < 2^(t - 1)
@@ -544,7 +549,8 @@ __device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint3
 // the current one and land in registers while they run (barriers wait only for
 // LDS), so the tile traffic overlaps the butterflies.  The stages are lazy: a
 // radix-4 pass always follows the head and takes values in [0, 4p).
-// Workgroup b's walk (b < 2^g); the caller has written L.r.
+// Workgroup b's walk (b < 2^g), HEAD_THREADS threads; the caller has written L.r.
+constexpr int HEAD_THREADS = 512;
 __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint32_t g,
                                           uint64_t b) {
   const uint32_t mbits = log_n - 2 * t;
@@ -567,34 +573,31 @@ __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_
   };
   uint64_t k = next(0);
   if (k >= K) return;
-  HeadStaged g0, g1, g2, g3;
+  static_assert(HEAD_W * HEAD_W == 2 * HEAD_THREADS, "two staged loads per thread");
+  HeadStaged g0, g1;
   auto fetch = [&](uint64_t M) {
     const uint64_t Mr = rev(M);
-    g0 = head_fetch(v, t, shift_a, M, Mr, 0);
-    g1 = head_fetch(v, t, shift_a, M, Mr, 1);
-    g2 = head_fetch(v, t, shift_a, M, Mr, 2);
-    g3 = head_fetch(v, t, shift_a, M, Mr, 3);
+    g0 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 0);
+    g1 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 1);
   };
   fetch(item(k));
   for (;;) {
     const uint64_t M = item(k), Mr = rev(M);
     const bool pair = M != Mr;
-    head_place(L, t, pair, 0, g0);
-    head_place(L, t, pair, 1, g1);
-    head_place(L, t, pair, 2, g2);
-    head_place(L, t, pair, 3, g3);
+    head_place<HEAD_THREADS>(L, t, pair, 0, g0);
+    head_place<HEAD_THREADS>(L, t, pair, 1, g1);
     __syncthreads();
     const uint64_t kn = next(k + 1);
     if (kn < K) fetch(item(kn));
-    head_stages<true>(L, t, pair);
-    head_store(L, v, t, shift_a, M, Mr, pair);
+    head_stages<true, HEAD_THREADS>(L, t, pair);
+    head_store<HEAD_THREADS>(L, v, t, shift_a, M, Mr, pair);
     if (kn >= K) break;
     __syncthreads();  // the store phase has read the tiles before they are replaced
     k = kn;
   }
 }
 
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(HEAD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     bitrev_head_stream(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t, uint32_t g) {
   __shared__ HeadLds L;
   head_twiddles_from_table(L, tw, log_n, t);
@@ -621,7 +624,7 @@ __global__ void __launch_bounds__(THREADS)
 // Fused bit reversal + first head_t stages of every instance; instance k owns
 // blocks [head_prefix[k], head_prefix[k + 1]) (2^head_g of them walking its tile
 // pairs as in bitrev_head_stream, or none when head_t == 0).
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(HEAD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     bitrev_head_batched(const BatchInstance* __restrict__ inst, const uint64_t* __restrict__ head_prefix,
                         uint32_t count) {
   __shared__ HeadLds L;
@@ -679,6 +682,7 @@ __device__ __forceinline__ uint64_t tile_elem(const TileGeom& g, uint32_t d, uin
 // one twiddle for both of its butterflies (k0 and k0+2h agree modulo 2^(s0+m)),
 // stage m+1 two.  Per LDS round trip a thread does 2 quads = 8 butterflies with
 // 4 independent products per stage; an odd last stage runs radix-2.
+template <int NT, int U>
 __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __restrict__ lds1,
                                             const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
                                             uint32_t c_log, const TileGeom& geo) {
@@ -691,13 +695,13 @@ __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __r
     const uint64_t mask1 = (1ull << s) - 1, mask2 = (1ull << (s + 1)) - 1;
     const uint32_t sh1 = log_n - 1 - s, sh2 = log_n - 2 - s;
     const uint32_t quads = T >> 2;
-    for (uint32_t g0 = threadIdx.x; g0 < quads; g0 += 2 * THREADS) {
-      Fr x[2][4], w1[2], w2[2], w3[2];
-      uint32_t pos[2][4];
-      bool live[2];
+    for (uint32_t g0 = threadIdx.x; g0 < quads; g0 += U * NT) {
+      Fr x[U][4], w1[U], w2[U], w3[U];
+      uint32_t pos[U][4];
+      bool live[U];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const uint32_t g = g0 + u * THREADS;
+      for (int u = 0; u < U; u++) {
+        const uint32_t g = g0 + u * NT;
         live[u] = g < quads;
         const uint32_t gg = live[u] ? g : g0;
         const uint32_t c = gg & (C - 1);
@@ -715,7 +719,7 @@ __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __r
         }
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
+      for (int u = 0; u < U; u++) {
         Fr t1, t3, t2, t4;
         Fr::mul_lazy2(x[u][1], w1[u], x[u][3], w1[u], t1, t3);
         const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
@@ -729,7 +733,7 @@ __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __r
         x[u][3] = Fr::sub_2p(a1n, t4);
       }
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
+      for (int u = 0; u < U; u++) {
         if (!live[u]) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) lds_put(lds0, lds1, pos[u][r], x[u][r]);
@@ -742,7 +746,7 @@ __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __r
     const uint32_t half = 1u << m;
     const uint64_t jmask = (1ull << s) - 1;
     const uint32_t tw_shift = log_n - 1 - s;
-    for (uint32_t b = threadIdx.x; b < (T >> 1); b += THREADS) {
+    for (uint32_t b = threadIdx.x; b < (T >> 1); b += NT) {
       const uint32_t c = b & (C - 1);
       const uint32_t q = b >> c_log;
       const uint32_t ka = ((q >> m) << (m + 1)) | (q & (half - 1));
@@ -760,11 +764,12 @@ __device__ __forceinline__ void pass_stages(uint4* __restrict__ lds0, uint4* __r
 
 // The tile back to memory; the transform's last pass (final) stores
 // canonical values.
+template <int NT>
 __device__ __forceinline__ void pass_store(const uint4* __restrict__ lds0, const uint4* __restrict__ lds1,
                                            const PolyView& v, const TileGeom& geo, uint32_t d, uint32_t c_log,
                                            bool final_pass) {
   const uint32_t T = 1u << (d + c_log);
-  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+  for (uint32_t e = threadIdx.x; e < T; e += NT) {
     uint32_t pos;
     uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_elem(geo, d, c_log, e, pos)));
     if (final_pass) {
@@ -778,32 +783,48 @@ __device__ __forceinline__ void pass_store(const uint4* __restrict__ lds0, const
   }
 }
 
-// One tile per workgroup: load, stages, store.
+// One tile per workgroup of NT threads (U quads per thread per round trip):
+// load, stages, store.
+template <int NT, int U>
 __device__ __forceinline__ void dit_tile_r4(uint4* __restrict__ lds0, uint4* __restrict__ lds1, const PolyView& v,
                                             const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
                                             uint32_t c_log, uint64_t tile, bool final_pass) {
   const uint32_t T = 1u << (d + c_log);
   const TileGeom geo = tile_geom(s0, d, c_log, tile);
-  for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+  for (uint32_t e = threadIdx.x; e < T; e += NT) {
     uint32_t pos;
     const uint4* src = reinterpret_cast<const uint4*>(v.elem(tile_elem(geo, d, c_log, e, pos)));
     lds0[pos] = src[0];
     lds1[pos] = src[1];
   }
   __syncthreads();
-  pass_stages(lds0, lds1, tw, log_n, s0, d, c_log, geo);
-  pass_store(lds0, lds1, v, geo, d, c_log, final_pass);
+  pass_stages<NT, U>(lds0, lds1, tw, log_n, s0, d, c_log, geo);
+  pass_store<NT>(lds0, lds1, v, geo, d, c_log, final_pass);
 }
 
-__global__ void __launch_bounds__(THREADS)
+// Passes run 512 threads per workgroup, one quad per thread per round trip:
+// with two workgroups per CU (64 KB of LDS each) that is four waves per SIMD in
+// at most 128 VGPRs, whose issue covers the waits (barriers, tile loads) that
+// two waves left exposed (2^24: 2.34 -> 2.18 ms).
+constexpr int PASS_THREADS = 512;
+__global__ void __launch_bounds__(PASS_THREADS)
     dit_pass_r4(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log,
                 uint32_t final_pass) {
   __shared__ uint4 lds0[1 << TILE_LOG];
   __shared__ uint4 lds1[1 << TILE_LOG];
-  dit_tile_r4(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x, final_pass != 0);
+  dit_tile_r4<PASS_THREADS, 1>(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x, final_pass != 0);
 }
 
+// The earlier shape: 256 threads, two quads each per round trip (variant 3).
 __global__ void __launch_bounds__(THREADS)
+    dit_pass_r4_narrow(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t s0, uint32_t d,
+                       uint32_t c_log, uint32_t final_pass) {
+  __shared__ uint4 lds0[1 << TILE_LOG];
+  __shared__ uint4 lds1[1 << TILE_LOG];
+  dit_tile_r4<THREADS, 2>(lds0, lds1, v, tw, log_n, s0, d, c_log, blockIdx.x, final_pass != 0);
+}
+
+__global__ void __launch_bounds__(PASS_THREADS)
     dit_pass_r4_batched(const BatchInstance* __restrict__ inst, const BatchPass* __restrict__ pass,
                         const uint64_t* __restrict__ tile_prefix, uint32_t count) {
   __shared__ uint4 lds0[1 << TILE_LOG];
@@ -812,7 +833,8 @@ __global__ void __launch_bounds__(THREADS)
   if (b >= tile_prefix[count]) return;
   const uint32_t k = find_instance(tile_prefix, count, b);
   const BatchPass p = pass[k];
-  dit_tile_r4(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log, b - tile_prefix[k], p.pad != 0);
+  dit_tile_r4<PASS_THREADS, 1>(lds0, lds1, inst[k].v, inst[k].tw, inst[k].log_n, p.s0, p.d, p.c_log,
+                               b - tile_prefix[k], p.pad != 0);
 }
 
 // ---------------------------------------------------------------- recompute NTT
@@ -1053,7 +1075,8 @@ struct PlanV2 {
 static uint32_t g_min_col_log[41] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
                                      2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
 // 0: persistent head + radix-4 passes; 1: plain bit reversal + radix-2 passes;
-// 2: as 0 with one tile pair per head workgroup (batch plans treat 2 as 0)
+// 2: as 0 with one tile pair per head workgroup; 3: as 0 with 256-thread
+// passes (batch plans treat 2 and 3 as 0)
 static int g_variant = 0;
 
 static void plan_v2(uint32_t log_n, PlanV2* p) {
@@ -1163,15 +1186,19 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
                            stream, v, twiddle, (uint32_t)log_len, plan.head_t);
       } else {
         const uint32_t g = head_stream_g((uint32_t)log_len - 2 * plan.head_t);
-        hipLaunchKernelGGL(bitrev_head_stream, dim3(1u << g), dim3(THREADS), 0, stream, v, twiddle,
+        hipLaunchKernelGGL(bitrev_head_stream, dim3(1u << g), dim3(HEAD_THREADS), 0, stream, v, twiddle,
                            (uint32_t)log_len, plan.head_t, g);
       }
     }
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
     for (int p = 0; p < plan.np; p++) {
       const PassPlan& pp = plan.pass[p];
-      hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)pp.tiles), dim3(THREADS), 0, stream, v, twiddle,
-                         (uint32_t)log_len, pp.s0, pp.d, pp.c_log, (uint32_t)(p + 1 == plan.np));
+      if (g_variant == 3)
+        hipLaunchKernelGGL(dit_pass_r4_narrow, dim3((unsigned)pp.tiles), dim3(THREADS), 0, stream, v, twiddle,
+                           (uint32_t)log_len, pp.s0, pp.d, pp.c_log, (uint32_t)(p + 1 == plan.np));
+      else
+        hipLaunchKernelGGL(dit_pass_r4, dim3((unsigned)pp.tiles), dim3(PASS_THREADS), 0, stream, v, twiddle,
+                           (uint32_t)log_len, pp.s0, pp.d, pp.c_log, (uint32_t)(p + 1 == plan.np));
       if ((err = hipGetLastError()) != hipSuccess || (err = mark(1)) != hipSuccess) return err;
     }
   }
@@ -1238,7 +1265,7 @@ hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned 
 // Not thread-safe against concurrent launches.  No counterpart in the
 // reference ABI (tuning hook).
 int zkp_ntt_set_variant(int variant) {
-  if (variant < 0 || variant > 2) return -1;
+  if (variant < 0 || variant > 3) return -1;
   const int old = zkp::ntt::g_variant;
   zkp::ntt::g_variant = variant;
   return old;
@@ -1501,7 +1528,7 @@ hipError_t zkp_ntt_batch_run(void* plan_ptr, hipStream_t stream) {
                        plan->d_inst, plan->d_elem_prefix, plan->count);
     err = hipGetLastError();
   } else if (plan->total_head_blocks > 0) {
-    hipLaunchKernelGGL(bitrev_head_batched, dim3((unsigned)plan->total_head_blocks), dim3(THREADS), 0, stream,
+    hipLaunchKernelGGL(bitrev_head_batched, dim3((unsigned)plan->total_head_blocks), dim3(HEAD_THREADS), 0, stream,
                        plan->d_inst, plan->d_elem_prefix, plan->count);
     err = hipGetLastError();
   }
@@ -1511,7 +1538,7 @@ hipError_t zkp_ntt_batch_run(void* plan_ptr, hipStream_t stream) {
       hipLaunchKernelGGL(dit_pass_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream,
                          plan->d_inst, plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
     else
-      hipLaunchKernelGGL(dit_pass_r4_batched, dim3((unsigned)plan->total_tiles[r]), dim3(THREADS), 0, stream,
+      hipLaunchKernelGGL(dit_pass_r4_batched, dim3((unsigned)plan->total_tiles[r]), dim3(PASS_THREADS), 0, stream,
                          plan->d_inst, plan->d_pass[r], plan->d_tile_prefix[r], plan->count);
     err = hipGetLastError();
   }
