@@ -376,11 +376,13 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         const uint32_t uu = live[u] ? uq : u0;
         const uint32_t tile = uu >= per_tile_q ? 1 : 0;
         const uint32_t r = uu - tile * per_tile_q;
-        const uint32_t a = r >> (t - 2);
-        const uint32_t q = r & ((W >> 2) - 1);
+        // consecutive lanes take consecutive LDS columns ra (the W independent
+        // transforms of a tile are interchangeable), so a row's accesses hit
+        // distinct banks
+        const uint32_t ra = r & (W - 1);
+        const uint32_t q = r >> t;
         const uint32_t k0 = ((q >> m) << (m + 2)) | (q & (h - 1));
         const uint32_t off = tile ? 0 : y0;
-        const uint32_t ra = rev_bits(a, t);
         jw[u] = k0 & (h - 1);
 #pragma unroll
         for (int r4 = 0; r4 < 4; r4++) {
@@ -396,13 +398,14 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         if (LAZY) {
           if (!skip1) {
             const Fr w1 = lds_get(L.r0, L.r1, jw[u] << (t - 1 - m));
-            Fr::mul_lazy2(x[u][1], w1, x[u][3], w1, t1, t3);
+            t1 = Fr::mul_lazy(x[u][1], w1);
+            t3 = Fr::mul_lazy(x[u][3], w1);
           }
           const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
           const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
           const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
-          Fr t2, t4;
-          Fr::mul_lazy2(a2, w2, a3, w3, t2, t4);
+          const Fr t2 = Fr::mul_lazy(a2, w2);
+          const Fr t4 = Fr::mul_lazy(a3, w3);
           const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
           x[u][0] = Fr::add_raw(a0n, t2);
           x[u][2] = Fr::sub_2p(a0n, t2);
@@ -447,11 +450,10 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
         const uint32_t uu = live[j] ? u : u0;
         const uint32_t tile = uu >= per_tile ? 1 : 0;
         const uint32_t r = uu - tile * per_tile;
-        const uint32_t a = r >> (t - 1);
-        const uint32_t q = r & ((W >> 1) - 1);
+        const uint32_t ra = r & (W - 1);  // consecutive lanes, consecutive LDS columns
+        const uint32_t q = r >> t;
         const uint32_t blo = ((q >> m) << (m + 1)) | (q & (half - 1));
         const uint32_t off = tile ? 0 : y0;
-        const uint32_t ra = rev_bits(a, t);
         pa[j] = off + rev_bits(blo, t) * HEAD_PITCH + ra;
         pb[j] = off + rev_bits(blo + half, t) * HEAD_PITCH + ra;
         wi[j] = (blo & (half - 1)) << (t - 1 - m);
@@ -550,7 +552,9 @@ __device__ __forceinline__ void head_twiddles_from_table(HeadLds& L, const uint3
 // LDS), so the tile traffic overlaps the butterflies.  The stages are lazy: a
 // radix-4 pass always follows the head and takes values in [0, 4p).
 // Workgroup b's walk (b < 2^g), HEAD_THREADS threads; the caller has written L.r.
-constexpr int HEAD_THREADS = 512;
+// (512 threads measured no faster: the staged loads of the next pair do not fit
+// the 128 VGPRs of four waves per SIMD and spill.)
+constexpr int HEAD_THREADS = 256;
 __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_t log_n, uint32_t t, uint32_t g,
                                           uint64_t b) {
   const uint32_t mbits = log_n - 2 * t;
@@ -573,12 +577,14 @@ __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_
   };
   uint64_t k = next(0);
   if (k >= K) return;
-  static_assert(HEAD_W * HEAD_W == 2 * HEAD_THREADS, "two staged loads per thread");
-  HeadStaged g0, g1;
+  static_assert(HEAD_W * HEAD_W == 4 * HEAD_THREADS, "four staged loads per thread");
+  HeadStaged g0, g1, g2, g3;
   auto fetch = [&](uint64_t M) {
     const uint64_t Mr = rev(M);
     g0 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 0);
     g1 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 1);
+    g2 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 2);
+    g3 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 3);
   };
   fetch(item(k));
   for (;;) {
@@ -586,6 +592,8 @@ __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_
     const bool pair = M != Mr;
     head_place<HEAD_THREADS>(L, t, pair, 0, g0);
     head_place<HEAD_THREADS>(L, t, pair, 1, g1);
+    head_place<HEAD_THREADS>(L, t, pair, 2, g2);
+    head_place<HEAD_THREADS>(L, t, pair, 3, g3);
     __syncthreads();
     const uint64_t kn = next(k + 1);
     if (kn < K) fetch(item(kn));
@@ -597,7 +605,7 @@ __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_
   }
 }
 
-__global__ void __launch_bounds__(HEAD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(HEAD_THREADS)
     bitrev_head_stream(PolyView v, const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t t, uint32_t g) {
   __shared__ HeadLds L;
   head_twiddles_from_table(L, tw, log_n, t);
@@ -624,7 +632,7 @@ __global__ void __launch_bounds__(THREADS)
 // Fused bit reversal + first head_t stages of every instance; instance k owns
 // blocks [head_prefix[k], head_prefix[k + 1]) (2^head_g of them walking its tile
 // pairs as in bitrev_head_stream, or none when head_t == 0).
-__global__ void __launch_bounds__(HEAD_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(HEAD_THREADS)
     bitrev_head_batched(const BatchInstance* __restrict__ inst, const uint64_t* __restrict__ head_prefix,
                         uint32_t count) {
   __shared__ HeadLds L;
