@@ -166,8 +166,9 @@ def headline(libs, stream, args):
 def suite(libs, stream, steps):
     """Config 2: forward / inverse / coset at 2^20, 2^22, 2^24 and the 4x extended coset of 2^22."""
     from zkpoly_amd import hip
-    from zkpoly_amd.field import mont_limbs
-    from zkpoly_amd.ntt import gen_powers
+    from zkpoly_amd.field import mont_limbs, omega
+    from zkpoly_amd.ntt import GenPqOmegas, gen_powers
+    from zkpoly_amd.runtime import ScalarArray
 
     out = {}
     for k in (20, 22, 24):
@@ -183,10 +184,26 @@ def suite(libs, stream, steps):
             w.lib.distribute_powers(w.ptr, pw.addr, n, stream.raw())
             w.ntt()
 
-        for name, fn in (("ntt", lambda: w.ntt()), ("intt", lambda: w.ntt(True)), ("coset_ntt", coset)):
+        # recompute_ntt: twiddles formed from the pq table and the omega ladder
+        # (no n/2 table), pq sized as the reference does (GenPqOmegas::get_pq_len)
+        pq_len = GenPqOmegas.get_pq_len(k)
+        pq = np.zeros((pq_len, 4), dtype=np.uint64)
+        om = np.zeros((32, 4), dtype=np.uint64)
+        GenPqOmegas(libs).get_fn()(ScalarArray(pq_len, pq.ctypes.data, device="cpu"),
+                                   ScalarArray(32, om.ctypes.data, device="cpu"), n, mont_limbs(omega(k)))
+        pqb, omb = hip.DeviceBuffer(pq.nbytes), hip.DeviceBuffer(om.nbytes)
+        pqb.upload(pq)
+        omb.upload(om)
+        pq_deg = pq_len.bit_length()
+
+        def recompute():
+            hip.check(w.lib.recompute_ntt(w.ptr, pqb.addr, pq_deg, omb.addr, k, stream.raw(), 8, 8), "recompute_ntt")
+
+        for name, fn in (("ntt", lambda: w.ntt()), ("intt", lambda: w.ntt(True)), ("coset_ntt", coset),
+                         ("recompute_ntt", recompute)):
             ms = time_loop(stream, fn, steps, 2)
             out[f"{name}_2^{k}"] = {"ms": round(ms, 4), "gelem_s": round(n / ms / 1e6, 3)}
-        del w, pw
+        del w, pw, pqb, omb
     # 4x extended coset of a 2^22 polynomial: zero-pad to 2^24, coset g = 7, NTT on 2^24
     w = Ntt(libs, stream, 24, seed=4422)
     n = w.n

@@ -67,7 +67,7 @@ def test_ssip_ntt_random_omega_rotated_slice(libs, stream, k):
     assert np.array_equal(got, want), f"k={k}"
 
 
-DEFAULT_MIN_COL_LOG = 2
+DEFAULT_MIN_COL_LOG = 1
 
 
 @pytest.mark.parametrize("variant,min_col", [(0, 1), (0, 3), (0, 6), (1, 2), (2, 2), (3, 1), (3, 3)])
@@ -112,10 +112,10 @@ def test_head_variants_at_large_sizes(libs, stream, k):
 
 def test_kernel_counts(libs):
     assert kernel_count(libs, 0) == 0
-    assert kernel_count(libs, 24) == 4  # fused head + 7 + 6 + 6 stages
-    set_tuning(libs, 24, 1)
+    assert kernel_count(libs, 24) == 3  # fused head + 10 + 9 stages (default min_col_log 1)
+    set_tuning(libs, 24, 2)
     try:
-        assert kernel_count(libs, 24) == 3  # fused head + 10 + 9 stages
+        assert kernel_count(libs, 24) == 4  # fused head + 7 + 6 + 6 stages
     finally:
         set_tuning(libs, 24, DEFAULT_MIN_COL_LOG)
     old = set_variant(libs, 1)

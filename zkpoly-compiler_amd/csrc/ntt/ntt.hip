@@ -881,34 +881,33 @@ __device__ __forceinline__ Fr pow_ladder(const uint32_t* __restrict__ omegas, ui
 }
 
 // r[j] = w^(j * 2^(log_n - D)) for j < 2^(D - 1): copied out of pq when it is
-// fine enough, else built by doubling from the ladder.  Block-cooperative; ends
-// with a barrier.
+// fine enough, else built by doubling from the ladder.  Block-cooperative (any
+// block size); ends with a barrier.
 __device__ void build_small_roots(uint4* r0, uint4* r1, const uint32_t* __restrict__ pq, uint32_t pq_deg,
                                   const uint32_t* __restrict__ omegas, uint32_t log_n, uint32_t D) {
   const uint32_t cnt = (1u << D) >> 1;
+  const uint32_t nt = blockDim.x;
   if (pq_deg >= D && pq_deg <= log_n) {
-    for (uint32_t j = threadIdx.x; j < cnt; j += THREADS) lds_put(r0, r1, j, Fr::load(pq + ((uint64_t)j << (pq_deg - D)) * 8));
+    for (uint32_t j = threadIdx.x; j < cnt; j += nt) lds_put(r0, r1, j, Fr::load(pq + ((uint64_t)j << (pq_deg - D)) * 8));
   } else {
     if (threadIdx.x == 0) lds_put(r0, r1, 0, Fr::one());
     for (uint32_t i = 0; (1u << i) < cnt; i++) {
       __syncthreads();
       const Fr g = Fr::load(omegas + (log_n - D + i) * 8);
-      for (uint32_t j = threadIdx.x; j < (1u << i); j += THREADS)
-        lds_put(r0, r1, j + (1u << i), lds_get(r0, r1, j) * g);
+      for (uint32_t j = threadIdx.x; j < (1u << i); j += nt) lds_put(r0, r1, j + (1u << i), lds_get(r0, r1, j) * g);
     }
   }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(THREADS)
-    rbitrev_head(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg, const uint32_t* __restrict__ omegas,
-                 uint32_t log_n, uint32_t t) {
+// The persistent head walk (bitrev_head_stream) with the head's twiddles built
+// from pq / the omega ladder.
+__global__ void __launch_bounds__(HEAD_THREADS)
+    rbitrev_head_stream(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg,
+                        const uint32_t* __restrict__ omegas, uint32_t log_n, uint32_t t, uint32_t g) {
   __shared__ HeadLds L;
-  const uint64_t M = blockIdx.x;
-  const uint32_t mbits = log_n - 2 * t;
-  if (mbits && M > (__brevll(M) >> (64 - mbits))) return;  // uniform per block
   build_small_roots(L.r0, L.r1, pq, pq_deg, omegas, log_n, t);
-  head_tile<false>(L, v, log_n, t, M);
+  head_walk(L, v, log_n, t, g, blockIdx.x);
 }
 
 struct RPassLds {
@@ -921,9 +920,14 @@ struct RPassLds {
 
 // Stages [s0, s0 + d) over tiles [blockIdx.x * per_block, ...) of 2^d rows x
 // 2^c_log columns, tile geometry as dit_tile.
-__global__ void __launch_bounds__(THREADS)
+// 512 threads (one butterfly each per stage at the 1024-element tile); lazy
+// butterflies, values in [0, 4p) between stages and passes, canonical stores
+// in the transform's last pass (final_pass).
+constexpr int RPASS_THREADS = 512;
+__global__ void __launch_bounds__(RPASS_THREADS)
     rdit_pass(PolyView v, const uint32_t* __restrict__ pq, uint32_t pq_deg, const uint32_t* __restrict__ omegas,
-              uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log, uint64_t tiles, uint32_t per_block) {
+              uint32_t log_n, uint32_t s0, uint32_t d, uint32_t c_log, uint64_t tiles, uint32_t per_block,
+              uint32_t final_pass) {
   __shared__ RPassLds S;
   const uint32_t C = 1u << c_log;
   const uint32_t R = 1u << d;
@@ -935,7 +939,7 @@ __global__ void __launch_bounds__(THREADS)
 
   if (s0) {
     const uint64_t lob = first % lo_blocks;
-    for (uint32_t e = threadIdx.x; e < C * d; e += THREADS) {
+    for (uint32_t e = threadIdx.x; e < C * d; e += RPASS_THREADS) {
       const uint32_t c = e / d, m = e - c * d;
       const uint32_t sh = log_n - 1 - s0 - m;
       lds_put(S.z0, S.z1, e, pow_ladder(omegas, c, sh));
@@ -958,7 +962,7 @@ __global__ void __launch_bounds__(THREADS)
       kstride = 1ull << s0;
       if (tile != first) {  // lo grows by C, or wraps to 0 with the next hi
         const bool wrap = lo_block == 0;
-        for (uint32_t e = threadIdx.x; e < C * d; e += THREADS) {
+        for (uint32_t e = threadIdx.x; e < C * d; e += RPASS_THREADS) {
           const uint32_t m = e % d;
           const Fr nb = wrap ? lds_get(S.z0, S.z1, e)
                              : lds_get(S.b0, S.b1, e) * Fr::load(omegas + (c_log + log_n - 1 - s0 - m) * 8);
@@ -966,7 +970,7 @@ __global__ void __launch_bounds__(THREADS)
         }
       }
     }
-    for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    for (uint32_t e = threadIdx.x; e < T; e += RPASS_THREADS) {
       uint32_t c, k;
       if (cstride == 1) {
         c = e & (C - 1);
@@ -983,47 +987,28 @@ __global__ void __launch_bounds__(THREADS)
     __syncthreads();
     for (uint32_t m = 0; m < d; m++) {
       const uint32_t h = 1u << m;
-      for (uint32_t e = threadIdx.x; e < (C << m); e += THREADS) {
+      for (uint32_t e = threadIdx.x; e < (C << m); e += RPASS_THREADS) {
         const uint32_t c = e >> m, kp = e & (h - 1);
         Fr w = lds_get(S.r0, S.r1, kp << (d - 1 - m));
         if (s0) w = lds_get(S.b0, S.b1, c * d + m) * w;
         lds_put(S.w0, S.w1, e, w);
       }
       __syncthreads();
-      for (uint32_t b0 = threadIdx.x; b0 < (T >> 1); b0 += 2 * THREADS) {
-        Fr x[2], y[2], w[2];
-        uint32_t pa[2], pb[2];
-        bool live[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const uint32_t b = b0 + u * THREADS;
-          live[u] = b < (T >> 1);
-          const uint32_t bb = live[u] ? b : b0;
-          const uint32_t c = bb & (C - 1);
-          const uint32_t q = bb >> c_log;
-          const uint32_t ka = ((q >> m) << (m + 1)) | (q & (h - 1));
-          pa[u] = (ka << c_log) | c;
-          pb[u] = ((ka + h) << c_log) | c;
-          w[u] = lds_get(S.w0, S.w1, (c << m) | (ka & (h - 1)));
-          x[u] = lds_get(S.x0, S.x1, pa[u]);
-          y[u] = lds_get(S.x0, S.x1, pb[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const Fr t = y[u] * w[u];
-          y[u] = x[u] - t;
-          x[u] = x[u] + t;
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          if (!live[u]) continue;
-          lds_put(S.x0, S.x1, pa[u], x[u]);
-          lds_put(S.x0, S.x1, pb[u], y[u]);
-        }
+      for (uint32_t b = threadIdx.x; b < (T >> 1); b += RPASS_THREADS) {
+        const uint32_t c = b & (C - 1);
+        const uint32_t q = b >> c_log;
+        const uint32_t ka = ((q >> m) << (m + 1)) | (q & (h - 1));
+        const uint32_t pa = (ka << c_log) | c;
+        const uint32_t pb = ((ka + h) << c_log) | c;
+        const Fr w = lds_get(S.w0, S.w1, (c << m) | (ka & (h - 1)));
+        const Fr x = Fr::reduce_2p(lds_get(S.x0, S.x1, pa));
+        const Fr t = Fr::mul_lazy(lds_get(S.x0, S.x1, pb), w);
+        lds_put(S.x0, S.x1, pa, Fr::add_raw(x, t));
+        lds_put(S.x0, S.x1, pb, Fr::sub_2p(x, t));
       }
       __syncthreads();
     }
-    for (uint32_t e = threadIdx.x; e < T; e += THREADS) {
+    for (uint32_t e = threadIdx.x; e < T; e += RPASS_THREADS) {
       uint32_t c, k;
       if (cstride == 1) {
         c = e & (C - 1);
@@ -1034,8 +1019,14 @@ __global__ void __launch_bounds__(THREADS)
       }
       uint4* dst = reinterpret_cast<uint4*>(v.elem(tile_index(base, cstride, kstride, c, k)));
       const uint32_t pos = (k << c_log) | c;
-      dst[0] = S.x0[pos];
-      dst[1] = S.x1[pos];
+      if (final_pass) {
+        const Fr y = Fr::reduce_4p(lds_get(S.x0, S.x1, pos));
+        dst[0] = make_uint4(y.v[0], y.v[1], y.v[2], y.v[3]);
+        dst[1] = make_uint4(y.v[4], y.v[5], y.v[6], y.v[7]);
+      } else {
+        dst[0] = S.x0[pos];
+        dst[1] = S.x1[pos];
+      }
     }
     __syncthreads();
   }
@@ -1079,9 +1070,10 @@ struct PlanV2 {
 };
 
 // per log_n minimum column exponent for v2 passes (runs of >= 2^c elements);
-// set by the autotuner through zkp_ntt_set_tuning
-static uint32_t g_min_col_log[41] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2,
-                                     2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+// set by the autotuner through zkp_ntt_set_tuning.  The default 1 (fewest,
+// deepest passes) measured fastest at 2^20..2^24 with 512-thread passes.
+static uint32_t g_min_col_log[41] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                     1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
 // 0: persistent head + radix-4 passes; 1: plain bit reversal + radix-2 passes;
 // 2: as 0 with one tile pair per head workgroup; 3: as 0 with 256-thread
 // passes (batch plans treat 2 and 3 as 0)
@@ -1325,8 +1317,9 @@ hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq, unsigned int pq_deg,
     hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
                        (uint32_t)log_len);
   } else {
-    hipLaunchKernelGGL(rbitrev_head, dim3((unsigned)(1ull << (log_len - 2 * plan.head_t))), dim3(THREADS), 0, stream,
-                       v, pq, (uint32_t)pq_deg, omegas, (uint32_t)log_len, plan.head_t);
+    const uint32_t g = head_stream_g((uint32_t)log_len - 2 * plan.head_t);
+    hipLaunchKernelGGL(rbitrev_head_stream, dim3(1u << g), dim3(HEAD_THREADS), 0, stream, v, pq, (uint32_t)pq_deg,
+                       omegas, (uint32_t)log_len, plan.head_t, g);
   }
   hipError_t err = hipGetLastError();
   for (int p = 0; p < plan.np && err == hipSuccess; p++) {
@@ -1334,8 +1327,8 @@ hipError_t recompute_ntt(PolyPtr x, const unsigned int* pq, unsigned int pq_deg,
     const uint64_t blocks = q.tiles < RBLOCKS ? q.tiles : RBLOCKS;
     const uint32_t per_block = (uint32_t)((q.tiles + blocks - 1) / blocks);
     const unsigned grid = (unsigned)((q.tiles + per_block - 1) / per_block);
-    hipLaunchKernelGGL(rdit_pass, dim3(grid), dim3(THREADS), 0, stream, v, pq, (uint32_t)pq_deg, omegas,
-                       (uint32_t)log_len, q.s0, q.d, q.c_log, q.tiles, per_block);
+    hipLaunchKernelGGL(rdit_pass, dim3(grid), dim3(RPASS_THREADS), 0, stream, v, pq, (uint32_t)pq_deg, omegas,
+                       (uint32_t)log_len, q.s0, q.d, q.c_log, q.tiles, per_block, (uint32_t)(p + 1 == plan.np));
     err = hipGetLastError();
   }
   return err;

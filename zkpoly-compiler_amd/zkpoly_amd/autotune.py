@@ -21,7 +21,7 @@ from typing import Callable, Iterable
 
 NTT_CANDIDATES = (1, 2, 3)
 MSM_WINDOWS = tuple(range(12, 19))
-DEFAULT_MIN_COL_LOG = 2
+DEFAULT_MIN_COL_LOG = 1
 DEFAULT_MSM_WINDOW = 16
 
 
