@@ -59,6 +59,10 @@ def lib() -> ctypes.CDLL:
         L.orc_best_multiexp.argtypes = [P, P, P, U64, I]
         L.orc_naive_msm.argtypes = [P, P, P, U64]
         L.orc_inner_product.argtypes = [P, P, P, U64]
+        L.orc_poly_eval.argtypes = [P, P, U64, P]
+        L.orc_kate_division.argtypes = [P, P, U64, P]
+        L.orc_batch_invert.argtypes = [P, U64]
+        L.orc_scan_mul.argtypes = [P, U64]
         _lib = L
     return _lib
 
@@ -275,3 +279,33 @@ def inner_product(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     out = np.zeros(4, dtype=np.uint64)
     lib().orc_inner_product(_p(out), _p(a), _p(b), a.shape[0])
     return out
+
+
+def poly_eval(a: np.ndarray, x) -> np.ndarray:
+    """sum_i a_i x^i (Horner from the top, core/tests/poly.rs:173-176)."""
+    a = _c(a).reshape(-1, 4)
+    out = np.zeros(4, dtype=np.uint64)
+    lib().orc_poly_eval(_p(out), _p(a), a.shape[0], _p(fe(x)))
+    return out
+
+
+def kate_division(a: np.ndarray, b) -> np.ndarray:
+    """halo2_proofs::arithmetic::kate_division; n outputs with q[n-1] = 0."""
+    a = _c(a).reshape(-1, 4)
+    q = np.zeros_like(a)
+    lib().orc_kate_division(_p(q), _p(a), a.shape[0], _p(fe(b)))
+    return q
+
+
+def batch_invert(a: np.ndarray) -> np.ndarray:
+    """Element inverses, zero kept (ff batch_invert)."""
+    x = _c(a).reshape(-1, 4).copy()
+    lib().orc_batch_invert(_p(x), x.shape[0])
+    return x
+
+
+def scan_mul(a: np.ndarray) -> np.ndarray:
+    """Inclusive prefix products."""
+    x = _c(a).reshape(-1, 4).copy()
+    lib().orc_scan_mul(_p(x), x.shape[0])
+    return x

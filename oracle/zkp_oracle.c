@@ -26,6 +26,11 @@
  *    offset = len - rotate: core/src/common/iter/src/iter.cuh:309-327.
  *  - curve y^2 = x^3 + 3 over Fq, identity affine (0,0):
  *    core/src/common/curve/src/bn254.cuh:11-18, curve_xyzz.cuh:204-210.
+ *  - polynomial helpers the reference's poly tests compare against
+ *    (core/tests/poly.rs): eval_polynomial by Horner from the top (:173-176),
+ *    halo2_proofs::arithmetic::kate_division (q_i = a_{i+1} + b q_{i+1} from
+ *    the top, n - 1 outputs; :240-244), ff's batch_invert (zeros skipped;
+ *    :464-470) and the running product of scan_mul (:408-411).
  *
  * How each piece is pinned (tests/test_oracle_*.py): field ops against the
  * reference's own known-answer vectors (core/src/common/mont/tests/main.cu)
@@ -406,6 +411,52 @@ void orc_distribute_powers(u64* x_raw, u64 n, const u64* powers_raw, u64 m) {
   fe* x = (fe*)x_raw;
   const fe* pw = (const fe*)powers_raw;
   for (u64 i = 0; i < n; i++) f_mul(F, &x[i], &x[i], &pw[i % m]);
+}
+
+/* sum_i a_i x^i by Horner from the top (core/tests/poly.rs:173-176) */
+void orc_poly_eval(u64* out, const u64* a_raw, u64 n, const u64* x_raw) {
+  const fparams* F = &FR;
+  const fe* a = (const fe*)a_raw;
+  fe acc;
+  memset(&acc, 0, sizeof(acc));
+  for (u64 i = n; i-- > 0;) {
+    f_mul(F, &acc, &acc, (const fe*)x_raw);
+    f_add(F, &acc, &acc, &a[i]);
+  }
+  memcpy(out, acc.v, 32);
+}
+
+/* halo2 kate_division: b' = -b; from the top, lead = a_j - tmp, q_{j-1} = lead,
+ * tmp = lead * b'; n - 1 outputs q_0..q_{n-2}; q_{n-1} is written as 0 */
+void orc_kate_division(u64* q_raw, const u64* a_raw, u64 n, const u64* b_raw) {
+  const fparams* F = &FR;
+  const fe* a = (const fe*)a_raw;
+  fe* q = (fe*)q_raw;
+  fe nb, tmp, lead;
+  f_neg(F, &nb, (const fe*)b_raw);
+  memset(&tmp, 0, sizeof(tmp));
+  if (n == 0) return;
+  memset(&q[n - 1], 0, sizeof(fe));
+  for (u64 j = n - 1; j >= 1; j--) {
+    f_sub(F, &lead, &a[j], &tmp);
+    q[j - 1] = lead;
+    f_mul(F, &tmp, &lead, &nb);
+  }
+}
+
+/* every element inverted by Fermat; zero stays zero (ff batch_invert semantics) */
+void orc_batch_invert(u64* x_raw, u64 n) {
+  const fparams* F = &FR;
+  fe* x = (fe*)x_raw;
+  for (u64 i = 0; i < n; i++)
+    if (!f_is_zero(&x[i])) f_inv(F, &x[i], &x[i]);
+}
+
+/* x_i = x_0 * ... * x_i in place */
+void orc_scan_mul(u64* x_raw, u64 n) {
+  const fparams* F = &FR;
+  fe* x = (fe*)x_raw;
+  for (u64 i = 1; i < n; i++) f_mul(F, &x[i], &x[i - 1], &x[i]);
 }
 
 /* twiddle table [1, w, ..., w^(len-1)] by running product (ssip_precompute,

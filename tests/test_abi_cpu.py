@@ -49,7 +49,7 @@ def exported(path):
 
 def test_headers_declare_something():
     secs = header_sections()
-    assert {s[1] for s in secs} >= {"libntt.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so",
+    assert {s[1] for s in secs} >= {"libntt.so", "libpoly.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so",
                                     "libzkp_fused_codegen.so", "libzkp_serve.so"}
     assert all(s[2] for s in secs)
 
@@ -213,3 +213,80 @@ def test_msm_config_extra_carry_window_keeps_reference_tables():
     for s in range(4, 25):
         assert lib.zkp_msm_config(s, -(-254 // s), out) == 1
         assert out[1] == -(-254 // s) and out[3] == 1
+
+
+def _poly():
+    from zkpoly_amd.poly import load_lib
+
+    return load_lib(Libs())
+
+
+def _a256(x):
+    return (x + 255) // 256 * 256
+
+
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 1 << 20, (1 << 24) + 5])
+def test_poly_buffer_size_queries(n):
+    """The NULL-buffer size protocol of core/src/poly.rs:265-280, 364-378,
+    413-427, 570-586, 631-647: the size comes back and nothing is launched.
+    Scans keep one 32-byte product (64-byte Horner pair for eval / kate) per
+    2048-element tile, twice; batched_invert adds the n prefix products."""
+    from zkpoly_amd.poly import KateDivision, PolyEval, PolyInvert, PolyScan
+
+    libs = Libs()
+    tiles = -(-n // 2048)
+    assert PolyScan(libs).get_buffer_size(n) == 2 * _a256(tiles * 32)
+    assert PolyEval(libs).get_buffer_size(n) == _a256(tiles * 64) + 256
+    assert PolyInvert(libs).get_buffer_size(n) == _a256(n * 32) + 2 * _a256(tiles * 32) + 2 * 256
+    if n & (n - 1) == 0:
+        k = n.bit_length() - 1
+        assert KateDivision(libs).get_buffer_size(k) == 2 * _a256(tiles * 64)
+
+
+def test_poly_entry_points_reject_bad_args_without_launch():
+    lib = _poly()
+    size = ctypes.c_ulong(0)
+    a16, a8 = PolyPtr(0x1000, 16, 0, 0, 16), PolyPtr(0x2000, 8, 0, 0, 8)
+    # the result must hold max(|a|, |b|) elements
+    assert lib.poly_add(a8, a16, a8, None) == HIP_INVALID_VALUE
+    assert lib.poly_sub(a8, a8, a16, None) == HIP_INVALID_VALUE
+    assert lib.poly_mul(PolyPtr(0, 16, 0, 0, 16), a16, a8, None) == HIP_INVALID_VALUE
+    # bad views: rotation on a slice, slice longer than its array
+    assert lib.poly_zero(PolyPtr(0x1000, 16, 3, 5, 32), None) == HIP_INVALID_VALUE
+    assert lib.poly_one_coef(PolyPtr(0x1000, 16, 0, 0, 8), None) == HIP_INVALID_VALUE
+    assert lib.poly_one_lagrange(PolyPtr(0, 16, 0, 0, 16), None) == HIP_INVALID_VALUE
+    # empty operands do nothing
+    assert lib.poly_add(PolyPtr(0, 0, 0, 0, 0), PolyPtr(0, 0, 0, 0, 0), PolyPtr(0, 0, 0, 0, 0), None) == 0
+    assert lib.poly_zero(PolyPtr(0, 0, 0, 0, 0), None) == 0
+    # size queries need somewhere to write the size
+    assert lib.scan_mul(None, None, a16, None) == HIP_INVALID_VALUE
+    assert lib.batched_invert(None, None, a16, None) == HIP_INVALID_VALUE
+    # kate_division: both views must hold 2^log_p elements, b non-null
+    assert lib.kate_division(0x9000, None, 4, a8, 0x3000, a16, None) == HIP_INVALID_VALUE
+    assert lib.kate_division(0x9000, None, 4, a16, 0x3000, a8, None) == HIP_INVALID_VALUE
+    assert lib.kate_division(0x9000, None, 4, a16, None, a16, None) == HIP_INVALID_VALUE
+    assert lib.kate_division(0x9000, None, 41, a16, 0x3000, a16, None) == HIP_INVALID_VALUE
+    # poly_eval needs res and x
+    assert lib.poly_eval(0x9000, None, a16, None, 0x3000, None) == HIP_INVALID_VALUE
+    assert lib.poly_eval(0x9000, None, a16, 0x3000, None, None) == HIP_INVALID_VALUE
+    assert lib.poly_eval(None, ctypes.byref(size), a16, None, None, None) == 0 and size.value > 0
+    # empty scans / inversions launch nothing
+    assert lib.scan_mul(0x9000, None, PolyPtr(0, 0, 0, 0, 0), None) == 0
+    assert lib.batched_invert(0x9000, None, PolyPtr(0, 0, 0, 0, 0), None) == 0
+    assert lib.inv_scalar(None, None) == HIP_INVALID_VALUE
+    assert lib.scalar_pow(None, 5, None) == HIP_INVALID_VALUE
+
+
+def test_poly_operators_refuse_host_arrays():
+    """The reference computes host-resident operands on the CPU; this package
+    has no CPU compute path and says so."""
+    import numpy as np
+
+    from zkpoly_amd.poly import PolyZero, ScalarInv
+    from zkpoly_amd.runtime import Scalar
+
+    host = np.zeros((4, 4), dtype=np.uint64)
+    with pytest.raises(NotImplementedError):
+        PolyZero(Libs()).get_fn().f([ScalarArray(4, host.ctypes.data, device="cpu")], [])
+    with pytest.raises(NotImplementedError):
+        ScalarInv(Libs()).get_fn().f([Scalar(host.ctypes.data, device="cpu")], [])

@@ -1,0 +1,463 @@
+// libpoly.so -- BN254 Fr polynomial kernels for gfx950.
+//
+// C ABI of the reference's libpoly.so (core/src/poly/src/poly.h:4-28;
+// cudaError_t -> hipError_t, cudaStream_t -> hipStream_t), bound by the
+// closures of core/src/poly.rs.  Semantics:
+//  * poly_add / poly_sub / poly_mul: element i < min(|a|, |b|) is a op b; the
+//    rest of the longer operand is copied (for a - b with |a| < |b|: negated)
+//    up to max(|a|, |b|) (poly_basic.cu:4-54, poly_basic.cuh:4-52);
+//  * poly_zero / poly_one_coef / poly_one_lagrange: 0..0 / 1,0..0 / 1..1;
+//  * inv_scalar (zero stays zero) / scalar_pow on one device element
+//    (poly_basic.cu:65-73);
+//  * scan_mul: in-place inclusive prefix product (scan_mul.cuh:7-27);
+//  * batched_invert: x_i^-1 for x_i != 0, zero stays zero
+//    (batched_invert.cuh:49-91);
+//  * poly_eval: sum_i a_i x^i into a device element (poly_eval.cuh:8-71);
+//  * kate_division: q_i = sum_{j>i} p_j b^(j-i-1), i.e. the quotient of
+//    (p(X) - p(b)) / (X - b), with q_{n-1} = 0 (kate_division.cuh:55-83).
+// Field arithmetic is exact, so any association order gives the reference's
+// bytes; the work is organised for the GPU rather than after cub.
+//
+// Scans and folds run over a monoid in three launches: every workgroup folds a
+// tile of TILE = 2048 elements (8 per thread, then an ordered LDS scan over the
+// 256 thread folds); one workgroup scans the tile folds into per-tile carries;
+// every workgroup rescans its tile seeded with its carry and emits results.
+// Monoids: products (scan_mul, batched_invert) and the Horner pair
+//   (S, M): (S_l, M_l) o (S_u, M_u) = (S_l + M_l S_u, M_l M_u),
+// whose fold over elements (a_i, x) in index order is (sum a_i x^i, x^n):
+// poly_eval is its total and kate_division its reverse exclusive scan over
+// (p_j, b).  A reverse scan runs forward over mirrored indices with the
+// combine's operands swapped.
+//
+// Temporary buffers follow the reference's protocol: with temp_buf == NULL
+// the size is written to *temp_buf_size and nothing runs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../common/poly_ptr.hpp"
+#include "../field/bn254.hpp"
+
+namespace zkp {
+namespace poly {
+
+constexpr int TH = 256;
+constexpr int PER = 8;
+constexpr int TILE = TH * PER;
+
+__device__ __forceinline__ Fr ld(const PolyView& v, uint64_t i) { return Fr::load(v.elem(i)); }
+__device__ __forceinline__ void st(const PolyView& v, uint64_t i, const Fr& x) { x.store(v.elem(i)); }
+
+// ---------------------------------------------------------------- elementwise
+
+// op: 0 add, 1 sub, 2 mul.  a is the longer operand; `neg` (sub with |a| < |b|
+// swapped) means the result is b - a on the common part and -a on the tail.
+__global__ void __launch_bounds__(TH)
+    k_binary(PolyView r, PolyView a, PolyView b, uint64_t len, uint64_t upper, int op, bool neg) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < upper; i += stride) {
+    const Fr x = ld(a, i);
+    Fr y;
+    if (i >= len) {
+      y = (op == 1 && neg) ? x.neg() : x;
+    } else {
+      const Fr z = ld(b, i);
+      y = op == 0 ? x + z : op == 2 ? x * z : (neg ? z - x : x - z);
+    }
+    st(r, i, y);
+  }
+}
+
+// mode: 0 all zero, 1 one then zeros, 2 all one
+__global__ void __launch_bounds__(TH) k_fill(PolyView r, int mode) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r.len; i += stride)
+    st(r, i, (mode == 2 || (mode == 1 && i == 0)) ? Fr::one() : Fr::zero());
+}
+
+__global__ void k_inv_scalar(uint32_t* x) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) Fr::load(x).inv().store(x);
+}
+
+__global__ void k_pow_scalar(uint32_t* x, uint64_t e) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) Fr::load(x).pow(e).store(x);
+}
+
+// ---------------------------------------------------------------- monoids
+
+struct ProdOp {
+  struct T {
+    Fr v;
+  };
+  __device__ static T id() { return T{Fr::one()}; }
+  __device__ static T comb(const T& l, const T& u) { return T{l.v * u.v}; }
+};
+
+struct HornerOp {
+  struct T {
+    Fr s, m;
+  };
+  __device__ static T id() { return T{Fr::zero(), Fr::one()}; }
+  __device__ static T comb(const T& l, const T& u) {
+    Fr ms, mm;
+    Fr::mul2(l.m, u.s, l.m, u.m, ms, mm);
+    return T{l.s + ms, mm};
+  }
+};
+
+// Reverse order: the same monoid with the operands of the combine swapped.
+template <class Op>
+struct RevOp {
+  using T = typename Op::T;
+  __device__ static T id() { return Op::id(); }
+  __device__ static T comb(const T& l, const T& u) { return Op::comb(u, l); }
+};
+
+// ---------------------------------------------------------------- scan machinery
+
+// Fold of elements [k0, min(k0 + PER, n)) in order.
+template <class Op, class Src>
+__device__ __forceinline__ typename Op::T thread_fold(const Src& src, uint64_t k0, uint64_t n) {
+  typename Op::T acc = Op::id();
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k >= n) break;
+    acc = Op::comb(acc, src(k));
+  }
+  return acc;
+}
+
+// Exclusive scan of one value per thread in thread order (Hillis-Steele in
+// LDS); *total = fold of all of them.  Every thread of the block calls it.
+template <class Op>
+__device__ __forceinline__ typename Op::T block_exclusive(const typename Op::T& v, typename Op::T* sh,
+                                                          typename Op::T* total) {
+  using T = typename Op::T;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < TH; off <<= 1) {
+    T cur = sh[threadIdx.x];
+    if (threadIdx.x >= off) cur = Op::comb(sh[threadIdx.x - off], cur);
+    __syncthreads();
+    sh[threadIdx.x] = cur;
+    __syncthreads();
+  }
+  const T excl = threadIdx.x ? sh[threadIdx.x - 1] : Op::id();
+  *total = sh[TH - 1];
+  __syncthreads();  // sh is reused by the caller
+  return excl;
+}
+
+// partial[b] = fold of tile b.
+template <class Op, class Src>
+__global__ void __launch_bounds__(TH) k_tile_fold(Src src, uint64_t n, typename Op::T* __restrict__ partial) {
+  __shared__ typename Op::T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  typename Op::T tot;
+  block_exclusive<Op>(thread_fold<Op>(src, k0, n), sh, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One workgroup: carry[b] = fold of partial[0..b) (when carry is given) and
+// *total = fold of all nb partials (when total is given).
+template <class Op>
+__global__ void __launch_bounds__(TH) k_partials_scan(const typename Op::T* __restrict__ partial, uint64_t nb,
+                                                      typename Op::T* __restrict__ carry,
+                                                      typename Op::T* __restrict__ total) {
+  using T = typename Op::T;
+  __shared__ T sh[TH];
+  const uint64_t per = (nb + TH - 1) / TH;
+  const uint64_t lo = (uint64_t)threadIdx.x * per;
+  const uint64_t hi = lo + per < nb ? lo + per : nb;
+  T v = Op::id();
+  for (uint64_t j = lo; j < hi; j++) v = Op::comb(v, partial[j]);
+  T tot;
+  T acc = block_exclusive<Op>(v, sh, &tot);
+  if (carry) {
+    for (uint64_t j = lo; j < hi; j++) {
+      carry[j] = acc;
+      acc = Op::comb(acc, partial[j]);
+    }
+  }
+  if (total && threadIdx.x == 0) *total = tot;
+}
+
+// Rescan of every tile seeded with its carry; sink(k, exclusive, element)
+// receives the fold of elements [0, k) and element k.
+template <class Op, class Src, class Sink>
+__global__ void __launch_bounds__(TH)
+    k_tile_scan(Src src, Sink sink, uint64_t n, const typename Op::T* __restrict__ carry) {
+  using T = typename Op::T;
+  __shared__ T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  T tot;
+  const T excl = block_exclusive<Op>(thread_fold<Op>(src, k0, n), sh, &tot);
+  T acc = Op::comb(carry[blockIdx.x], excl);
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k >= n) break;
+    const T e = src(k);
+    sink(k, acc, e);
+    acc = Op::comb(acc, e);
+  }
+}
+
+static inline uint64_t tiles_of(uint64_t n) { return (n + TILE - 1) / TILE; }
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Runs the three phases of a scan of n > 0 elements; partial / carry hold
+// tiles_of(n) entries each; total (optional) receives the fold of everything.
+template <class Op, class Src, class Sink>
+static hipError_t run_scan(const Src& src, const Sink& sink, uint64_t n, typename Op::T* partial,
+                           typename Op::T* carry, typename Op::T* total, hipStream_t stream) {
+  const uint64_t nb = tiles_of(n);
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_tile_fold<Op, Src>), dim3((unsigned)nb), dim3(TH), 0, stream, src, n, partial);
+  hipLaunchKernelGGL((k_partials_scan<Op>), dim3(1), dim3(TH), 0, stream, partial, nb, carry, total);
+  hipLaunchKernelGGL((k_tile_scan<Op, Src, Sink>), dim3((unsigned)nb), dim3(TH), 0, stream, src, sink, n, carry);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- element sources and sinks
+
+struct SrcProd {  // x_k
+  PolyView v;
+  __device__ ProdOp::T operator()(uint64_t k) const { return ProdOp::T{ld(v, k)}; }
+};
+struct SinkInclusive {  // x_k = prefix product through k
+  PolyView v;
+  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T& e) const {
+    st(v, k, excl.v * e.v);
+  }
+};
+
+struct SrcNonZero {  // x_k, or 1 for x_k = 0
+  PolyView v;
+  __device__ ProdOp::T operator()(uint64_t k) const {
+    const Fr x = ld(v, k);
+    return ProdOp::T{x.is_zero() ? Fr::one() : x};
+  }
+};
+struct SrcNonZeroRev {  // x_{n-1-k}, or 1 for zero
+  PolyView v;
+  __device__ ProdOp::T operator()(uint64_t k) const {
+    const Fr x = ld(v, v.len - 1 - k);
+    return ProdOp::T{x.is_zero() ? Fr::one() : x};
+  }
+};
+struct SinkStoreExcl {  // pre[k] = product of the non-zero x_j, j < k
+  Fr* pre;
+  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&) const { pre[k] = excl.v; }
+};
+struct SinkInvert {  // x_i = pre[i] * suf[i] / total for x_i != 0 (i = n-1-k)
+  PolyView v;
+  const Fr* pre;
+  const Fr* inv_total;
+  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&) const {
+    const uint64_t i = v.len - 1 - k;
+    const Fr x = ld(v, i);
+    st(v, i, x.is_zero() ? Fr::zero() : pre[i] * excl.v * (*inv_total));
+  }
+};
+
+struct SrcHorner {  // (a_k, x)
+  PolyView v;
+  const uint32_t* x;
+  __device__ HornerOp::T operator()(uint64_t k) const { return HornerOp::T{ld(v, k), Fr::load(x)}; }
+};
+struct SrcHornerRev {  // (p_{n-1-k}, b)
+  PolyView v;
+  const uint32_t* b;
+  __device__ HornerOp::T operator()(uint64_t k) const { return HornerOp::T{ld(v, v.len - 1 - k), Fr::load(b)}; }
+};
+struct SinkKate {  // q_{n-1-k} = S of the fold of (p_j, b), j > n-1-k
+  PolyView q;
+  __device__ void operator()(uint64_t k, const HornerOp::T& excl, const HornerOp::T&) const {
+    st(q, q.len - 1 - k, excl.s);
+  }
+};
+
+__global__ void k_invert_total(const Fr* total, Fr* inv) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *inv = total->inv();
+}
+
+__global__ void k_store_s(const HornerOp::T* total, uint32_t* res) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) total->s.store(res);
+}
+
+static inline unsigned grid_for(uint64_t n) {
+  uint64_t g = (n + TH - 1) / TH;
+  if (g > 65536) g = 65536;
+  return g ? (unsigned)g : 1u;
+}
+
+}  // namespace poly
+}  // namespace zkp
+
+using namespace zkp;
+using namespace zkp::poly;
+
+// ---------------------------------------------------------------- C ABI
+
+static hipError_t binary(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream, int op) {
+  PolyView vr, va, vb;
+  if (!make_view(r, &vr) || !make_view(a, &va) || !make_view(b, &vb)) return hipErrorInvalidValue;
+  bool neg = false;
+  if (va.len < vb.len) {
+    const PolyView t = va;
+    va = vb;
+    vb = t;
+    neg = true;  // only sub uses it
+  }
+  const uint64_t len = vb.len, upper = va.len;
+  if (upper == 0) return hipSuccess;
+  if (vr.len < upper || vr.ptr == nullptr || va.ptr == nullptr || (len && vb.ptr == nullptr))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_binary, dim3(grid_for(upper)), dim3(TH), 0, stream, vr, va, vb, len, upper, op,
+                     op == 1 && neg);
+  return hipGetLastError();
+}
+
+static hipError_t fill(PolyPtr r, hipStream_t stream, int mode) {
+  PolyView v;
+  if (!make_view(r, &v)) return hipErrorInvalidValue;
+  if (v.len == 0) return hipSuccess;
+  if (v.ptr == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(v.len)), dim3(TH), 0, stream, v, mode);
+  return hipGetLastError();
+}
+
+extern "C" {
+
+// r = a + b (poly.h:4)
+hipError_t poly_add(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) { return binary(r, a, b, stream, 0); }
+
+// r = a - b (poly.h:6)
+hipError_t poly_sub(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) { return binary(r, a, b, stream, 1); }
+
+// r = a * b elementwise (poly.h:8)
+hipError_t poly_mul(PolyPtr r, ConstPolyPtr a, ConstPolyPtr b, hipStream_t stream) { return binary(r, a, b, stream, 2); }
+
+// every element one (poly.h:10)
+hipError_t poly_one_lagrange(PolyPtr target, hipStream_t stream) { return fill(target, stream, 2); }
+
+// 1, 0, ..., 0 (poly.h:12)
+hipError_t poly_one_coef(PolyPtr target, hipStream_t stream) { return fill(target, stream, 1); }
+
+// every element zero (poly.h:14)
+hipError_t poly_zero(PolyPtr target, hipStream_t stream) { return fill(target, stream, 0); }
+
+// *res = sum_i poly_i x^i; res and x are device elements (poly.h:16)
+hipError_t poly_eval(void* temp_buf, unsigned long* temp_buf_size, ConstPolyPtr poly, unsigned int* res,
+                     const unsigned int* x, hipStream_t stream) {
+  PolyView v;
+  if (!make_view(poly, &v)) return hipErrorInvalidValue;
+  const uint64_t nb = tiles_of(v.len ? v.len : 1);
+  const size_t part = align256(nb * sizeof(HornerOp::T));
+  if (temp_buf == nullptr) {
+    if (temp_buf_size == nullptr) return hipErrorInvalidValue;
+    *temp_buf_size = part + align256(sizeof(HornerOp::T));
+    return hipSuccess;
+  }
+  if (res == nullptr || x == nullptr) return hipErrorInvalidValue;
+  if (v.len == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(TH), 0, stream, PolyView{res, 1, 0, 1}, 0);
+    return hipGetLastError();
+  }
+  if (v.ptr == nullptr) return hipErrorInvalidValue;
+  auto* partial = reinterpret_cast<HornerOp::T*>(temp_buf);
+  auto* total = reinterpret_cast<HornerOp::T*>(reinterpret_cast<char*>(temp_buf) + part);
+  const SrcHorner src{v, x};
+  hipLaunchKernelGGL((k_tile_fold<HornerOp, SrcHorner>), dim3((unsigned)nb), dim3(TH), 0, stream, src, v.len,
+                     partial);
+  hipLaunchKernelGGL((k_partials_scan<HornerOp>), dim3(1), dim3(TH), 0, stream, partial, nb,
+                     (HornerOp::T*)nullptr, total);
+  hipLaunchKernelGGL(k_store_s, dim3(1), dim3(1), 0, stream, total, res);
+  return hipGetLastError();
+}
+
+// q = (p(X) - p(b)) / (X - b) for p of 2^log_p coefficients, q_{n-1} = 0
+// (poly.h:18)
+hipError_t kate_division(void* temp_buf, unsigned long* temp_buf_size, unsigned int log_p, ConstPolyPtr p,
+                         const unsigned int* b, PolyPtr q, hipStream_t stream) {
+  if (log_p > 40) return hipErrorInvalidValue;
+  const uint64_t n = 1ull << log_p;
+  const uint64_t nb = tiles_of(n);
+  const size_t part = align256(nb * sizeof(HornerOp::T));
+  if (temp_buf == nullptr) {
+    if (temp_buf_size == nullptr) return hipErrorInvalidValue;
+    *temp_buf_size = 2 * part;
+    return hipSuccess;
+  }
+  PolyView vp, vq;
+  if (!make_view(p, &vp) || !make_view(q, &vq) || vp.len != n || vq.len != n) return hipErrorInvalidValue;
+  if (vp.ptr == nullptr || vq.ptr == nullptr || b == nullptr) return hipErrorInvalidValue;
+  using Op = RevOp<HornerOp>;
+  auto* partial = reinterpret_cast<HornerOp::T*>(temp_buf);
+  auto* carry = reinterpret_cast<HornerOp::T*>(reinterpret_cast<char*>(temp_buf) + part);
+  return run_scan<Op>(SrcHornerRev{vp, b}, SinkKate{vq}, n, partial, carry, (HornerOp::T*)nullptr, stream);
+}
+
+// target_i = target_0 * ... * target_i in place (poly.h:20)
+hipError_t scan_mul(void* temp_buffer, unsigned long* buffer_size, PolyPtr target, hipStream_t stream) {
+  PolyView v;
+  if (!make_view(target, &v)) return hipErrorInvalidValue;
+  const uint64_t nb = tiles_of(v.len ? v.len : 1);
+  const size_t part = align256(nb * sizeof(ProdOp::T));
+  if (temp_buffer == nullptr) {
+    if (buffer_size == nullptr) return hipErrorInvalidValue;
+    *buffer_size = 2 * part;
+    return hipSuccess;
+  }
+  if (v.len == 0) return hipSuccess;
+  if (v.ptr == nullptr) return hipErrorInvalidValue;
+  auto* partial = reinterpret_cast<ProdOp::T*>(temp_buffer);
+  auto* carry = reinterpret_cast<ProdOp::T*>(reinterpret_cast<char*>(temp_buffer) + part);
+  return run_scan<ProdOp>(SrcProd{v}, SinkInclusive{v}, v.len, partial, carry, (ProdOp::T*)nullptr, stream);
+}
+
+// poly_i = poly_i^-1 in place, zeros kept (poly.h:22)
+hipError_t batched_invert(void* temp_buffer, unsigned long* buffer_size, PolyPtr poly, hipStream_t stream) {
+  PolyView v;
+  if (!make_view(poly, &v)) return hipErrorInvalidValue;
+  const uint64_t n = v.len;
+  const uint64_t nb = tiles_of(n ? n : 1);
+  const size_t pre = align256((n ? n : 1) * sizeof(Fr));
+  const size_t part = align256(nb * sizeof(ProdOp::T));
+  const size_t one = align256(sizeof(Fr));
+  if (temp_buffer == nullptr) {
+    if (buffer_size == nullptr) return hipErrorInvalidValue;
+    *buffer_size = pre + 2 * part + 2 * one;
+    return hipSuccess;
+  }
+  if (n == 0) return hipSuccess;
+  if (v.ptr == nullptr) return hipErrorInvalidValue;
+  char* t = reinterpret_cast<char*>(temp_buffer);
+  Fr* prefix = reinterpret_cast<Fr*>(t);
+  auto* partial = reinterpret_cast<ProdOp::T*>(t + pre);
+  auto* carry = reinterpret_cast<ProdOp::T*>(t + pre + part);
+  auto* total = reinterpret_cast<ProdOp::T*>(t + pre + 2 * part);
+  Fr* inv = reinterpret_cast<Fr*>(t + pre + 2 * part + one);
+  // prefix[i] = product of the non-zero x_j, j < i; total = product of all of them
+  hipError_t err = run_scan<ProdOp>(SrcNonZero{v}, SinkStoreExcl{prefix}, n, partial, carry, total, stream);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_invert_total, dim3(1), dim3(1), 0, stream, &total->v, inv);
+  // the suffix products (mirrored scan) finish x_i^-1 = prefix_i * suffix_i / total
+  return run_scan<ProdOp>(SrcNonZeroRev{v}, SinkInvert{v, prefix, inv}, n, partial, carry, (ProdOp::T*)nullptr,
+                          stream);
+}
+
+// *target = *target^-1, zero stays zero (poly.h:24)
+hipError_t inv_scalar(unsigned int* target, hipStream_t stream) {
+  if (target == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_inv_scalar, dim3(1), dim3(1), 0, stream, target);
+  return hipGetLastError();
+}
+
+// *target = *target^exp (poly.h:26)
+hipError_t scalar_pow(unsigned int* target, unsigned long long exp, hipStream_t stream) {
+  if (target == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pow_scalar, dim3(1), dim3(1), 0, stream, target, (uint64_t)exp);
+  return hipGetLastError();
+}
+
+}  // extern "C"

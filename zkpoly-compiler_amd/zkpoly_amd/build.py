@@ -73,6 +73,10 @@ def build_ntt() -> str:
     return build_shared([os.path.join(CSRC, "ntt", "ntt.hip")], "libntt.so")
 
 
+def build_poly() -> str:
+    return build_shared([os.path.join(CSRC, "poly", "poly.hip")], "libpoly.so")
+
+
 def build_msm_core() -> str:
     return build_shared([os.path.join(CSRC, "msm", "msm.hip")], "libzkp_msm.so")
 
@@ -152,7 +156,7 @@ def build_intpeak() -> str:
 
 
 def targets() -> list:
-    return [build_ntt, build_msm_core, build_fused_codegen, build_intpeak]
+    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_intpeak]
 
 
 def build_all(jobs: int | None = None) -> list[str]:

@@ -81,6 +81,20 @@ class KernelType(enum.Enum):
     DistributePowers = "DistributePowers"
     Msm = "Msm"
     FusedArith = "FusedArith"
+    # libpoly.so operators (core/src/poly.rs); ScalarPow carries its exponent
+    # in FuncMeta.extra
+    PolyAdd = "PolyAdd"
+    PolySub = "PolySub"
+    NewZero = "NewZero"
+    NewOneLagrange = "NewOneLagrange"
+    NewOneCoef = "NewOneCoef"
+    EvaluatePoly = "EvaluatePoly"
+    KateDivision = "KateDivision"
+    ScanMul = "ScanMul"
+    BatchedInvert = "BatchedInvert"
+    ScalarInvert = "ScalarInvert"
+    ScalarPow = "ScalarPow"
+    Other = "Other"
 
 
 @dataclass
