@@ -66,6 +66,16 @@ hipError_t inv_scalar(unsigned int* target, hipStream_t stream);
 /* *target = *target^exp on the device.  Replaces scalar_pow, poly.h:26. */
 hipError_t scalar_pow(unsigned int* target, unsigned long long exp, hipStream_t stream);
 
+/* Lookup permutation of the first `usable` rows: res_input = input sorted by
+ * canonical value; res_table holds, at each row that starts a run of equal
+ * values, that value (one copy leaves the table multiset), and in the other
+ * rows the table's leftover values ascending.  input and table have equal
+ * lengths >= usable; res_input and res_table hold usable elements.  A value
+ * missing from the table keeps its own row (the reference asserts instead).
+ * Replaces permute, poly.h:28 (permute.cuh:84-221). */
+hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long usable, ConstPolyPtr input,
+                   ConstPolyPtr table, PolyPtr res_input, PolyPtr res_table, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

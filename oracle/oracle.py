@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
         L.orc_kate_division.argtypes = [P, P, U64, P]
         L.orc_batch_invert.argtypes = [P, U64]
         L.orc_scan_mul.argtypes = [P, U64]
+        L.orc_permute.argtypes = [P, P, P, P, U64]
         _lib = L
     return _lib
 
@@ -309,3 +310,14 @@ def scan_mul(a: np.ndarray) -> np.ndarray:
     x = _c(a).reshape(-1, 4).copy()
     lib().orc_scan_mul(_p(x), x.shape[0])
     return x
+
+
+def permute(inp: np.ndarray, table: np.ndarray, usable: int) -> tuple[np.ndarray, np.ndarray]:
+    """halo2 lookup permutation of the first `usable` rows (core/tests/poly.rs:635-690)."""
+    a = _c(inp).reshape(-1, 4)
+    t = _c(table).reshape(-1, 4)
+    assert a.shape[0] >= usable and t.shape[0] >= usable
+    out_in = np.zeros((usable, 4), dtype=np.uint64)
+    out_tab = np.zeros((usable, 4), dtype=np.uint64)
+    lib().orc_permute(_p(out_in), _p(out_tab), _p(a), _p(t), usable)
+    return out_in, out_tab

@@ -30,7 +30,8 @@
  *    (core/tests/poly.rs): eval_polynomial by Horner from the top (:173-176),
  *    halo2_proofs::arithmetic::kate_division (q_i = a_{i+1} + b q_{i+1} from
  *    the top, n - 1 outputs; :240-244), ff's batch_invert (zeros skipped;
- *    :464-470) and the running product of scan_mul (:408-411).
+ *    :464-470), the running product of scan_mul (:408-411) and the lookup
+ *    permutation of test_poly_permute (:635-690).
  *
  * How each piece is pinned (tests/test_oracle_*.py): field ops against the
  * reference's own known-answer vectors (core/src/common/mont/tests/main.cu)
@@ -457,6 +458,55 @@ void orc_scan_mul(u64* x_raw, u64 n) {
   const fparams* F = &FR;
   fe* x = (fe*)x_raw;
   for (u64 i = 1; i < n; i++) f_mul(F, &x[i], &x[i - 1], &x[i]);
+}
+
+/* halo2 permute_expression_pair as the reference's test restates it
+ * (core/tests/poly.rs:635-690): A' = first `usable` inputs sorted by canonical
+ * value; a row that starts a run of equal values in A' takes that value in S'
+ * and one copy of it leaves the table multiset; the remaining table values,
+ * ascending, fill the other rows in order.  An input value missing from the
+ * table (the reference stops there) keeps its own value and takes no copy out;
+ * the other rows then take the smallest leftovers. */
+static int cmp_canon(const void* pa, const void* pb) {
+  const u64* a = (const u64*)pa;
+  const u64* b = (const u64*)pb;
+  for (int i = 3; i >= 0; i--)
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  return 0;
+}
+
+void orc_permute(u64* out_in_raw, u64* out_tab_raw, const u64* in_raw, const u64* tab_raw, u64 usable) {
+  const fparams* F = &FR;
+  fe* sa = (fe*)malloc((usable ? usable : 1) * sizeof(fe));
+  fe* st = (fe*)malloc((usable ? usable : 1) * sizeof(fe));
+  unsigned char* used = (unsigned char*)calloc(usable ? usable : 1, 1);
+  fe* out_in = (fe*)out_in_raw;
+  fe* out_tab = (fe*)out_tab_raw;
+  for (u64 i = 0; i < usable; i++) {
+    f_to_canonical(F, &sa[i], (const fe*)in_raw + i);
+    f_to_canonical(F, &st[i], (const fe*)tab_raw + i);
+  }
+  qsort(sa, usable, sizeof(fe), cmp_canon);
+  qsort(st, usable, sizeof(fe), cmp_canon);
+  /* run starts: match one table copy each (both lists ascending) */
+  u64 t = 0;
+  for (u64 i = 0; i < usable; i++) {
+    if (i > 0 && cmp_canon(&sa[i], &sa[i - 1]) == 0) continue;
+    while (t < usable && cmp_canon(&st[t], &sa[i]) < 0) t++;
+    if (t < usable && cmp_canon(&st[t], &sa[i]) == 0) used[t++] = 1;
+    f_from_canonical(F, &out_tab[i], &sa[i]);
+  }
+  /* leftovers, ascending, into the repeated rows in row order */
+  u64 next = 0;
+  for (u64 i = 0; i < usable; i++) {
+    if (i == 0 || cmp_canon(&sa[i], &sa[i - 1]) != 0) continue;
+    while (used[next]) next++;
+    f_from_canonical(F, &out_tab[i], &st[next++]);
+  }
+  for (u64 i = 0; i < usable; i++) f_from_canonical(F, &out_in[i], &sa[i]);
+  free(sa);
+  free(st);
+  free(used);
 }
 
 /* twiddle table [1, w, ..., w^(len-1)] by running product (ssip_precompute,

@@ -290,3 +290,25 @@ def test_poly_operators_refuse_host_arrays():
         PolyZero(Libs()).get_fn().f([ScalarArray(4, host.ctypes.data, device="cpu")], [])
     with pytest.raises(NotImplementedError):
         ScalarInv(Libs()).get_fn().f([Scalar(host.ctypes.data, device="cpu")], [])
+
+
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, (1 << 20) - 25])
+def test_permute_buffer_size_and_rejects(n):
+    """Sizing as core/src/poly.rs:264-280 calls it: two key buffers for each of
+    input and table, the slot and rank flags, and the count scan's two
+    per-tile arrays."""
+    from zkpoly_amd.poly import PolyPermute
+
+    tiles = -(-n // 2048)
+    assert PolyPermute(Libs()).get_buffer_size(n) == 4 * _a256(n * 32) + 2 * _a256(n * 4) + 2 * _a256(tiles * 8)
+    lib = _poly()
+    whole, part = PolyPtr(0x1000, 64, 0, 0, 64), PolyPtr(0x2000, 60, 0, 0, 60)
+    # input and table of one length; results exactly `usable` long; usable <= len
+    assert lib.permute(0x9000, None, 60, whole, PolyPtr(0x3000, 63, 0, 0, 63), part, part, None) == HIP_INVALID_VALUE
+    assert lib.permute(0x9000, None, 60, whole, whole, whole, part, None) == HIP_INVALID_VALUE
+    assert lib.permute(0x9000, None, 64, part, part, whole, whole, None) == HIP_INVALID_VALUE
+    assert lib.permute(0x9000, None, 60, whole, whole, PolyPtr(0, 60, 0, 0, 60), part, None) == HIP_INVALID_VALUE
+    assert lib.permute(0x9000, None, 1 << 32, whole, whole, part, part, None) == HIP_INVALID_VALUE
+    assert lib.permute(None, None, 60, whole, whole, part, part, None) == HIP_INVALID_VALUE
+    empty = PolyPtr(0, 0, 0, 0, 0)
+    assert lib.permute(0x9000, None, 0, whole, whole, empty, empty, None) == 0

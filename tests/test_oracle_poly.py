@@ -81,3 +81,66 @@ def test_scan_mul_running_product():
 def test_mont_array_round_trip():
     xs = [0, 1, 5, P - 1]
     assert vals(mont_array(xs)) == xs
+
+
+def _perm(inputs, table, usable=None):
+    usable = len(inputs) if usable is None else usable
+    a, s = orc.permute(mont_array(inputs), mont_array(table), usable)
+    return vals(a), vals(s)
+
+
+def test_permute_worked_example():
+    """Worked by hand: runs of A' = [1, 2,2,2, 3,3] start at rows 0, 1, 4 and take
+    1, 2, 3 out of the table {1,2,2,3,4,5}; leftovers 2, 4, 5 fill rows 2, 3, 5."""
+    a, s = _perm([3, 1, 3, 2, 2, 2], [1, 2, 3, 4, 5, 2])
+    assert a == [1, 2, 2, 2, 3, 3]
+    assert s == [1, 2, 2, 4, 3, 5]
+
+
+def _check_lookup_permutation(inp, tab, a, s):
+    """What halo2's lookup argument requires of (A', S'), plus the leftover order
+    that makes the pair unique: A' is the input sorted; S' a permutation of the
+    table; each row has A'_i = S'_i or A'_i = A'_{i-1}; run starts hold their
+    own value; the other rows hold the leftovers in ascending order."""
+    n = len(a)
+    assert a == sorted(inp)
+    assert sorted(s) == sorted(tab)
+    starts = [i == 0 or a[i] != a[i - 1] for i in range(n)]
+    for i in range(n):
+        assert a[i] == s[i] or (i > 0 and a[i] == a[i - 1])
+        if starts[i]:
+            assert s[i] == a[i]
+    rest = [s[i] for i in range(n) if not starts[i]]
+    assert rest == sorted(rest)
+
+
+@pytest.mark.parametrize("n,spread,seed", [(1, 1, 1), (50, 8, 2), (300, 40, 3), (1000, 1000, 4), (257, 2, 5)])
+def test_permute_properties_small_values(n, spread, seed):
+    rng = np.random.default_rng(seed)
+    tab = [int(x) for x in rng.integers(0, spread, n)]
+    distinct = sorted(set(tab))
+    inp = [distinct[int(i)] for i in rng.integers(0, len(distinct), n)]
+    a, s = _perm(inp, tab)
+    _check_lookup_permutation(inp, tab, a, s)
+
+
+def test_permute_properties_field_elements_and_usable_prefix():
+    """Values spread over the whole field (negatives p - x share their top bits),
+    and only the first `usable` rows count."""
+    rng = np.random.default_rng(9)
+    n, usable = 600, 575
+    pool = [int(x) for x in rng.integers(0, 1 << 62, 40)] + [P - int(x) for x in rng.integers(1, 1 << 20, 40)]
+    tab = [pool[int(i)] for i in rng.integers(0, len(pool), n)]
+    head = sorted(set(tab[:usable]))
+    inp = [head[int(i)] for i in rng.integers(0, len(head), n)]
+    a, s = _perm(inp, tab, usable)
+    assert len(a) == len(s) == usable
+    _check_lookup_permutation(inp[:usable], tab[:usable], a, s)
+
+
+def test_permute_value_missing_from_table():
+    """A missing value keeps its own row and takes nothing from the table; the
+    repeated rows take the smallest leftovers."""
+    a, s = _perm([7, 7, 1], [1, 2, 3])
+    assert a == [1, 7, 7]
+    assert s == [1, 7, 2]

@@ -288,3 +288,124 @@ def test_pow_scalar(libs, stream, exp):
     f([z], [stream])
     stream.sync()
     assert np.array_equal(read_scalar(zb), mont_limbs(1 if exp == 0 else 0))
+
+
+# ---------------------------------------------------------------- permute
+
+R2 = orc.fe((1 << 512) % FR_MODULUS)
+
+
+def mont_small(xs):
+    """Montgomery form of small non-negative ints (< 2^64), vectorised: x * R^2 * R^-1."""
+    limbs = np.zeros((len(xs), 4), dtype=np.uint64)
+    limbs[:, 0] = np.asarray(xs, dtype=np.uint64)
+    return orc.vec("orc_vec_mul", FR, limbs, np.broadcast_to(R2, limbs.shape).copy())
+
+
+def run_permute(libs, stream, inp, tab, usable, views=None):
+    """views: optional (rotate_in, rotate_tab, slice_res) for the GPU arrays."""
+    from zkpoly_amd.poly import PolyPermute
+
+    rot_in, rot_tab, slice_res = views or (0, 0, False)
+    ib, iv = put(inp, rotate=rot_in)
+    tb_, tv = put(tab, rotate=rot_tab)
+    if slice_res:
+        rib, riw = device_array(usable + 1000)
+        riw.rotate(700)
+        ri = riw.slice(300, 300 + usable)
+        rtb, rtw = device_array(usable + 999)
+        rt = rtw.slice(999, 999 + usable)
+    else:
+        rib, ri = empty(usable)
+        rtb, rt = empty(usable)
+    op = PolyPermute(libs)
+    tmpb, tmp = temp_buffer(op.get_buffer_size(usable))
+    op.get_fn().f([ri, rt, tmp], [iv, tv, stream])
+    stream.sync()
+    return gpu2cpu(ri, rib), gpu2cpu(rt, rtb)
+
+
+def check_permute(libs, stream, inp, tab, usable, views=None):
+    got_in, got_tab = run_permute(libs, stream, inp, tab, usable, views)
+    want_in, want_tab = orc.permute(inp, tab, usable)
+    assert np.array_equal(got_in, want_in), "permuted input"
+    assert np.array_equal(got_tab, want_tab), "permuted table"
+
+
+def lookup_case(n, usable, seed, spread):
+    """core/tests/poly.rs:569-624: table values x mod spread, input drawn from
+    the distinct values of the table's usable rows."""
+    rng = np.random.default_rng(seed)
+    tab = rng.integers(0, spread, n, dtype=np.uint64)
+    distinct = np.unique(tab[:usable])
+    inp = distinct[rng.integers(0, len(distinct), n)]
+    return mont_small(inp), mont_small(tab)
+
+
+def test_permute_reference_case(libs, stream):
+    """core/tests/poly.rs:569-765: K = 20, 24 blinding rows + 1, table values
+    below 2 * len, input sampled from the usable table's distinct values."""
+    n = 1 << K
+    usable = n - 25
+    inp, tab = lookup_case(n, usable, 1, 2 * n)
+    check_permute(libs, stream, inp, tab, usable)
+
+
+@pytest.mark.parametrize("usable", [1, 2, 7, 2047, 2048, 2049, 4097, 3 * 2048 + 5, 100003, (1 << 21) + 3])
+def test_permute_sizes(libs, stream, usable):
+    n = usable + 3
+    inp, tab = lookup_case(n, usable, usable, max(2, usable // 3))
+    check_permute(libs, stream, inp, tab, usable)
+
+
+def test_permute_field_elements_with_negatives(libs, stream):
+    """Compressed lookup expressions look like random field elements; negatives
+    p - x share their top bits: the sort compares whole 256-bit values."""
+    n = 300000
+    rng = np.random.default_rng(5)
+    pool = np.concatenate([random_elements(4000, 6), orc.vec("orc_vec_neg", FR, mont_small(rng.integers(1, 1 << 40, 4000)))])
+    tab = pool[rng.integers(0, len(pool), n)]
+    distinct = np.unique(tab, axis=0)
+    inp = distinct[rng.integers(0, len(distinct), n)]
+    check_permute(libs, stream, inp, tab, n)
+
+
+def test_permute_views_and_degenerate_values(libs, stream):
+    n = 50000
+    inp, tab = lookup_case(n, n, 7, 1000)
+    check_permute(libs, stream, inp, tab, n, views=(123, n - 5, True))
+    # every value equal
+    same = mont_small(np.full(n, 42))
+    check_permute(libs, stream, same, same, n)
+    # all distinct: the permutation is the sorted table twice
+    tab = mont_small(np.random.default_rng(8).permutation(n))
+    got_in, got_tab = run_permute(libs, stream, tab[::-1].copy(), tab, n)
+    assert np.array_equal(got_in, got_tab) and np.array_equal(got_in, orc.permute(tab, tab, n)[0])
+
+
+def test_permute_value_missing_from_table(libs, stream):
+    """Beyond the reference (which asserts on the device): a missing value keeps
+    its row and the other rows take the smallest leftovers, as the oracle says."""
+    n = 5000
+    inp, tab = lookup_case(n, n, 9, 300)
+    inp[::97] = mont_small([10 ** 6])  # not in the table
+    check_permute(libs, stream, inp, tab, n)
+
+
+def test_permute_reuses_temp_for_a_smaller_call(libs, stream):
+    from zkpoly_amd.poly import PolyPermute
+
+    op = PolyPermute(libs)
+    big, small = 70000, 3000
+    tmpb, tmp = temp_buffer(op.get_buffer_size(big))
+    for usable, seed in ((big, 11), (small, 12)):
+        inp, tab = lookup_case(usable, usable, seed, usable // 2)
+        ib, iv = put(inp)
+        tb_, tv = put(tab)
+        rib, ri = empty(usable)
+        rtb, rt = empty(usable)
+        op.get_fn().f([ri, rt, tmp], [iv, tv, stream])
+        stream.sync()
+        want_in, want_tab = orc.permute(inp, tab, usable)
+        assert np.array_equal(gpu2cpu(ri, rib), want_in)
+        assert np.array_equal(gpu2cpu(rt, rtb), want_tab)

@@ -1,0 +1,139 @@
+// Shared machinery of libpoly.so: the element helpers and the three-launch
+// monoid scan (tile fold, one-workgroup scan of the tile folds, seeded tile
+// rescan) used by the scans, folds and the permutation.
+//
+// A monoid Op supplies T, id() and comb(l, u) (l precedes u).  A source
+// src(k) yields element k as an Op::T; a sink(k, exclusive, element) receives
+// the fold of elements [0, k) and element k.  Every workgroup folds TILE =
+// TH * PER consecutive elements (PER per thread, then an ordered LDS scan over
+// the TH thread folds).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../common/poly_ptr.hpp"
+#include "../field/bn254.hpp"
+
+namespace zkp {
+namespace poly {
+
+constexpr int TH = 256;
+constexpr int PER = 8;
+constexpr int TILE = TH * PER;
+
+__device__ __forceinline__ Fr ld(const PolyView& v, uint64_t i) { return Fr::load(v.elem(i)); }
+__device__ __forceinline__ void st(const PolyView& v, uint64_t i, const Fr& x) { x.store(v.elem(i)); }
+
+// ---------------------------------------------------------------- scan machinery
+
+// Fold of elements [k0, min(k0 + PER, n)) in order.
+template <class Op, class Src>
+__device__ __forceinline__ typename Op::T thread_fold(const Src& src, uint64_t k0, uint64_t n) {
+  typename Op::T acc = Op::id();
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k >= n) break;
+    acc = Op::comb(acc, src(k));
+  }
+  return acc;
+}
+
+// Exclusive scan of one value per thread in thread order (Hillis-Steele in
+// LDS); *total = fold of all of them.  Every thread of the block calls it.
+template <class Op>
+__device__ __forceinline__ typename Op::T block_exclusive(const typename Op::T& v, typename Op::T* sh,
+                                                          typename Op::T* total) {
+  using T = typename Op::T;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < TH; off <<= 1) {
+    T cur = sh[threadIdx.x];
+    if (threadIdx.x >= off) cur = Op::comb(sh[threadIdx.x - off], cur);
+    __syncthreads();
+    sh[threadIdx.x] = cur;
+    __syncthreads();
+  }
+  const T excl = threadIdx.x ? sh[threadIdx.x - 1] : Op::id();
+  *total = sh[TH - 1];
+  __syncthreads();  // sh is reused by the caller
+  return excl;
+}
+
+// partial[b] = fold of tile b.
+template <class Op, class Src>
+__global__ void __launch_bounds__(TH) k_tile_fold(Src src, uint64_t n, typename Op::T* __restrict__ partial) {
+  __shared__ typename Op::T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  typename Op::T tot;
+  block_exclusive<Op>(thread_fold<Op>(src, k0, n), sh, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One workgroup: carry[b] = fold of partial[0..b) (when carry is given) and
+// *total = fold of all nb partials (when total is given).
+template <class Op>
+__global__ void __launch_bounds__(TH) k_partials_scan(const typename Op::T* __restrict__ partial, uint64_t nb,
+                                                      typename Op::T* __restrict__ carry,
+                                                      typename Op::T* __restrict__ total) {
+  using T = typename Op::T;
+  __shared__ T sh[TH];
+  const uint64_t per = (nb + TH - 1) / TH;
+  const uint64_t lo = (uint64_t)threadIdx.x * per;
+  const uint64_t hi = lo + per < nb ? lo + per : nb;
+  T v = Op::id();
+  for (uint64_t j = lo; j < hi; j++) v = Op::comb(v, partial[j]);
+  T tot;
+  T acc = block_exclusive<Op>(v, sh, &tot);
+  if (carry) {
+    for (uint64_t j = lo; j < hi; j++) {
+      carry[j] = acc;
+      acc = Op::comb(acc, partial[j]);
+    }
+  }
+  if (total && threadIdx.x == 0) *total = tot;
+}
+
+// Rescan of every tile seeded with its carry; sink(k, exclusive, element)
+// receives the fold of elements [0, k) and element k.
+template <class Op, class Src, class Sink>
+__global__ void __launch_bounds__(TH)
+    k_tile_scan(Src src, Sink sink, uint64_t n, const typename Op::T* __restrict__ carry) {
+  using T = typename Op::T;
+  __shared__ T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  T tot;
+  const T excl = block_exclusive<Op>(thread_fold<Op>(src, k0, n), sh, &tot);
+  T acc = Op::comb(carry[blockIdx.x], excl);
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k >= n) break;
+    const T e = src(k);
+    sink(k, acc, e);
+    acc = Op::comb(acc, e);
+  }
+}
+
+static inline uint64_t tiles_of(uint64_t n) { return (n + TILE - 1) / TILE; }
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Runs the three phases of a scan of n > 0 elements; partial / carry hold
+// tiles_of(n) entries each; total (optional) receives the fold of everything.
+template <class Op, class Src, class Sink>
+static hipError_t run_scan(const Src& src, const Sink& sink, uint64_t n, typename Op::T* partial,
+                           typename Op::T* carry, typename Op::T* total, hipStream_t stream) {
+  const uint64_t nb = tiles_of(n);
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_tile_fold<Op, Src>), dim3((unsigned)nb), dim3(TH), 0, stream, src, n, partial);
+  hipLaunchKernelGGL((k_partials_scan<Op>), dim3(1), dim3(TH), 0, stream, partial, nb, carry, total);
+  hipLaunchKernelGGL((k_tile_scan<Op, Src, Sink>), dim3((unsigned)nb), dim3(TH), 0, stream, src, sink, n, carry);
+  return hipGetLastError();
+}
+
+static inline unsigned grid_for(uint64_t n) {
+  uint64_t g = (n + TH - 1) / TH;
+  if (g > 65536) g = 65536;
+  return g ? (unsigned)g : 1u;
+}
+
+}  // namespace poly
+}  // namespace zkp

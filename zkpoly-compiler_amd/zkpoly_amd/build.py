@@ -74,7 +74,8 @@ def build_ntt() -> str:
 
 
 def build_poly() -> str:
-    return build_shared([os.path.join(CSRC, "poly", "poly.hip")], "libpoly.so")
+    return build_shared([os.path.join(CSRC, "poly", "poly.hip"), os.path.join(CSRC, "poly", "permute.hip")],
+                        "libpoly.so")
 
 
 def build_msm_core() -> str:

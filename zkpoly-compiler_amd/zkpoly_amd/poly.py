@@ -12,6 +12,8 @@ Same operator names, argument lists and checks as the reference's wrappers
                 target = x0, x0 p_0, x0 p_0 p_1, ...; get_buffer_size(len)
   PolyInvert:   mut_var=[temp_buf, target], var=[stream]; get_buffer_size(len)
   ScalarInv / ScalarPow(exp): mut_var=[scalar], var=[stream]
+  PolyPermute:  mut_var=[res_input, res_table, temp_buf], var=[input, table, stream];
+                usable = |res_input| = |res_table|; get_buffer_size(usable)
 Temporary buffers are runtime.GpuBuffer objects sized by get_buffer_size
 (the library's own size query: a NULL buffer returns the size, runs nothing).
 
@@ -57,8 +59,9 @@ def load_lib(libs: Libs) -> ctypes.CDLL:
         lib.batched_invert.argtypes = [vp, ulp, PolyPtr, st]
         lib.inv_scalar.argtypes = [vp, st]
         lib.scalar_pow.argtypes = [vp, ctypes.c_ulonglong, st]
+        lib.permute.argtypes = [vp, ulp, ctypes.c_ulong, ConstPolyPtr, ConstPolyPtr, PolyPtr, PolyPtr, st]
         for name in ("poly_add", "poly_sub", "poly_mul", "poly_one_lagrange", "poly_one_coef", "poly_zero",
-                     "poly_eval", "kate_division", "scan_mul", "batched_invert", "inv_scalar", "scalar_pow"):
+                     "poly_eval", "kate_division", "scan_mul", "batched_invert", "inv_scalar", "scalar_pow", "permute"):
             getattr(lib, name).restype = ctypes.c_int
         lib._zkp_typed = True
     return lib
@@ -338,5 +341,35 @@ class ScalarPow(_PolyFunc):
         return Function(FuncMeta("scalar_pow", KernelType.ScalarPow, exp), f)
 
 
+class PolyPermute(_PolyFunc):
+    """Lookup permutation (A', S') of the first `usable` rows of (input, table)
+    (core/src/poly.rs:221-280): A' sorted; S' holds each run start's value and
+    the table's leftovers, ascending, in the other rows."""
+
+    SYMBOL = "permute"
+
+    def get_buffer_size(self, usable_len: int) -> int:
+        n = null_ptr(usable_len)
+        return _query(self.c_func, usable_len, n, n, n, n, None)
+
+    def get_fn(self) -> Function:
+        c_func = self.c_func
+
+        def f(mut_var, var, gpu_mapping=None):
+            assert len(mut_var) == 3
+            assert len(var) == 3
+            res_input = _gpu_array(mut_var[0], "permute")
+            res_table = _gpu_array(mut_var[1], "permute")
+            temp = unwrap(mut_var[2], GpuBuffer, "GpuBuffer")
+            inp = _gpu_array(var[0], "permute")
+            table = _gpu_array(var[1], "permute")
+            stream = unwrap_stream(var[2])
+            hip.set_device(stream.get_device())
+            hip.check(c_func(temp.ptr, None, res_input.len, PolyPtr.from_array(inp), PolyPtr.from_array(table),
+                             PolyPtr.from_array(res_input), PolyPtr.from_array(res_table), stream.raw()), "permute")
+
+        return Function(FuncMeta("poly_permute", KernelType.PolyPermute), f)
+
+
 __all__ = ["PolyAdd", "PolySub", "PolyMul", "PolyZero", "PolyOneLagrange", "PolyOneCoef", "PolyEval",
-           "KateDivision", "PolyScan", "PolyInvert", "ScalarInv", "ScalarPow", "load_lib", "null_ptr", "sub_view"]
+           "KateDivision", "PolyScan", "PolyInvert", "ScalarInv", "ScalarPow", "PolyPermute", "load_lib", "null_ptr", "sub_view"]

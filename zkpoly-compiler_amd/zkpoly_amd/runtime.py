@@ -94,6 +94,7 @@ class KernelType(enum.Enum):
     BatchedInvert = "BatchedInvert"
     ScalarInvert = "ScalarInvert"
     ScalarPow = "ScalarPow"
+    PolyPermute = "PolyPermute"
     Other = "Other"
 
 
