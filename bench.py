@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--serving-only", action="store_true", help="config-5 serving leg only")
     ap.add_argument("--serve-count", type=int, default=60, help="requests per rate in the serving leg")
     ap.add_argument("--msm-log-n", type=int, default=20)
+    ap.add_argument("--poly-only", action="store_true", help="libpoly.so leg only")
     return ap.parse_args()
 
 
@@ -380,6 +381,98 @@ def serving_leg(libs, stream, rates, count, with_cpu):
     return out
 
 
+# Per element of each libpoly entry point, the algorithmic minimum: HBM bytes
+# (each operand read once, each result written once) and Montgomery products
+# (the sequential recurrences: Horner 1, running product 1, Montgomery's batch
+# inversion trick 3; 136 multiply-adds each).  The parallel scans do more
+# (a tile fold and a rescan), which shows as a lower fraction.
+POLY_COSTS = {
+    "poly_add": (96, 0),
+    "poly_mul": (96, 1),
+    "poly_eval": (32, 1),
+    "kate_division": (64, 1),
+    "scan_mul": (64, 1),
+    "batched_invert": (64, 3),
+    "permute": (128, 0),
+}
+
+
+def poly_leg(libs, stream, log_n, steps, with_cpu):
+    """libpoly.so at 2^log_n resident elements: ms, Gelem/s, HBM and multiply fractions."""
+    from zkpoly_amd import hip
+    from zkpoly_amd.field import random_elements
+    from zkpoly_amd.poly import load_lib
+    from zkpoly_amd.runtime import PolyPtr
+
+    lib = load_lib(libs)
+    n = 1 << log_n
+    a, b, r = hip.DeviceBuffer(n * 32), hip.DeviceBuffer(n * 32), hip.DeviceBuffer(n * 32)
+    ha = random_elements(n, 70 + log_n)
+    a.upload(ha)
+    b.upload(random_elements(n, 80 + log_n))
+    x = hip.DeviceBuffer(32)
+    x.upload(random_elements(1, 90))
+    res = hip.DeviceBuffer(32)
+    pa, pb, pr = (PolyPtr(buf.addr, n, 0, 0, n) for buf in (a, b, r))
+    size = ctypes.c_ulong(0)
+
+    def temp(fn, *args):
+        hip.check(fn(None, ctypes.byref(size), *args), "size query")
+        return hip.DeviceBuffer(max(1, size.value))
+
+    t_eval = temp(lib.poly_eval, pa, None, None, None)
+    t_kate = temp(lib.kate_division, log_n, pa, None, pr, None)
+    t_scan = temp(lib.scan_mul, pr, None)
+    t_inv = temp(lib.batched_invert, pr, None)
+    # lookup columns: a table of n/2 distinct random elements with repeats, inputs drawn from it
+    rng = np.random.default_rng(91)
+    pool = random_elements(n // 2, 92)
+    tab_h = pool[rng.integers(0, n // 2, n)]
+    inp_h = tab_h[rng.integers(0, n, n)]
+    ti, tt, ri2 = hip.DeviceBuffer(n * 32), hip.DeviceBuffer(n * 32), hip.DeviceBuffer(n * 32)
+    ti.upload(inp_h)
+    tt.upload(tab_h)
+    del pool, tab_h, inp_h
+    pti, ptt, pri2 = (PolyPtr(buf.addr, n, 0, 0, n) for buf in (ti, tt, ri2))
+    t_perm = temp(lambda t_, s_: lib.permute(t_, s_, n, pti, ptt, pr, pri2, None))
+    s = stream.raw()
+    ops = {
+        "poly_add": lambda: hip.check(lib.poly_add(pr, pa, pb, s), "poly_add"),
+        "poly_mul": lambda: hip.check(lib.poly_mul(pr, pa, pb, s), "poly_mul"),
+        "poly_eval": lambda: hip.check(lib.poly_eval(t_eval.addr, None, pa, res.addr, x.addr, s), "poly_eval"),
+        "kate_division": lambda: hip.check(lib.kate_division(t_kate.addr, None, log_n, pa, x.addr, pr, s), "kate"),
+        "scan_mul": lambda: hip.check(lib.scan_mul(t_scan.addr, None, pr, s), "scan_mul"),
+        "batched_invert": lambda: hip.check(lib.batched_invert(t_inv.addr, None, pr, s), "batched_invert"),
+        "permute": lambda: hip.check(lib.permute(t_perm.addr, None, n, pti, ptt, pr, pri2, s), "permute"),
+    }
+    out = {"log_n": log_n, "ops": {}}
+    for name, fn in ops.items():
+        if name in ("scan_mul", "batched_invert"):
+            hip.check(lib.poly_add(pr, pa, PolyPtr(None, 0, 0, 0, 0), s), "copy")  # fresh non-zero input
+        ms = time_loop(stream, fn, steps, 2)
+        nbytes, products = POLY_COSTS[name]
+        gbs = nbytes * n / (ms * 1e-3) / 1e9
+        mads = products * MADS_PER_MODMUL * n / (ms * 1e-3)
+        out["ops"][name] = {
+            "ms": round(ms, 4),
+            "gelem_s": round(n / ms / 1e6, 3),
+            "hbm_gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "int_mul_frac": round(mads / INT_MUL_PEAK, 4),
+        }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        m = min(n, 1 << 22)
+        t = time.perf_counter()
+        orc.kate_division(ha[:m], random_elements(1, 90)[0])
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline_kate_division"] = {"value": round(m / cpu_s / 1e9, 5), "unit": "Gelem/s", "cores": 1,
+                                             "kind": "port", "sample": f"kate_division of 2^{m.bit_length() - 1}"}
+    return out
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -427,6 +520,10 @@ def main():
     if args.batched_only:
         log("[bench] batched form only")
         print(json.dumps({"batched": batched_leg(libs, stream, args.steps, not args.no_cpu)}), flush=True)
+        return
+    if args.poly_only:
+        log(f"[bench] libpoly.so 2^{args.log_n} only")
+        print(json.dumps({"poly": poly_leg(libs, stream, args.log_n, args.steps, not args.no_cpu)}), flush=True)
         return
     if args.msm_only:
         log(f"[bench] MSM 2^{args.msm_log_n} only")
@@ -508,6 +605,8 @@ def main():
         line["suite"] = suite(libs, stream, max(5, args.steps // 2))
         log("[bench] MSM 2^20 window sweep")
         line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
+        log("[bench] libpoly.so at 2^24")
+        line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
         line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
         log("[bench] serving form (Poisson 50..2000 req/s)")

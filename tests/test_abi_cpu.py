@@ -295,12 +295,13 @@ def test_poly_operators_refuse_host_arrays():
 @pytest.mark.parametrize("n", [1, 2047, 2048, 2049, (1 << 20) - 25])
 def test_permute_buffer_size_and_rejects(n):
     """Sizing as core/src/poly.rs:264-280 calls it: two key buffers for each of
-    input and table, the slot and rank flags, and the count scan's two
-    per-tile arrays."""
+    input and table, the slot and rank flags, the count scan's two per-tile
+    arrays and the merge splits (one per 2048-key tile of each array)."""
     from zkpoly_amd.poly import PolyPermute
 
     tiles = -(-n // 2048)
-    assert PolyPermute(Libs()).get_buffer_size(n) == 4 * _a256(n * 32) + 2 * _a256(n * 4) + 2 * _a256(tiles * 8)
+    want = 4 * _a256(n * 32) + 2 * _a256(n * 4) + 2 * _a256(tiles * 8) + _a256(2 * tiles * 4)
+    assert PolyPermute(Libs()).get_buffer_size(n) == want
     lib = _poly()
     whole, part = PolyPtr(0x1000, 64, 0, 0, 64), PolyPtr(0x2000, 60, 0, 0, 60)
     # input and table of one length; results exactly `usable` long; usable <= len

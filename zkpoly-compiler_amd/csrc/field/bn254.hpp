@@ -41,6 +41,8 @@ struct FrParams {
                                  0x7879462eu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
   static constexpr u32 R2[8] = {0xae216da7u, 0x1bb8e645u, 0xe35c59e3u, 0x53fe3ab1u,
                                 0x53bb8085u, 0x8c49833du, 0x7f4e44a5u, 0x0216d0b1u};
+  static constexpr u32 R3[8] = {0xb4bf0040u, 0x5e94d8e1u, 0x1cfbb6b8u, 0x2a489cbeu,
+                                0xa19fcfedu, 0x893cc664u, 0x7fcc657cu, 0x0cf8594bu};  // 2^768 mod r
 };
 
 struct FqParams {
@@ -52,6 +54,8 @@ struct FqParams {
                                  0x7879462cu, 0x666ea36fu, 0x9a07df2fu, 0x0e0a77c1u};
   static constexpr u32 R2[8] = {0x538afa89u, 0xf32cfc5bu, 0xd44501fbu, 0xb5e71911u,
                                 0x0a417ff6u, 0x47ab1effu, 0xcab8351fu, 0x06d89f71u};
+  static constexpr u32 R3[8] = {0xda1530dfu, 0xb1cd6dafu, 0xa7283db6u, 0x62f210e6u,
+                                0x0ada0afbu, 0xef7f0b0cu, 0x2d592544u, 0x20fd6e90u};  // 2^768 mod q
 };
 
 template <class Params>
@@ -476,6 +480,71 @@ struct Fp {
       }
     }
     return r;
+  }
+
+  // Inverse by the binary extended Euclidean algorithm, for a lone thread
+  // inverting one value (variable time: a few hundred 256-bit shifts and
+  // subtractions in place of Fermat's ~380 serial products).  The integer
+  // inverse of the Montgomery word aR is a^-1 R^-1; one product with R^3 gives
+  // a^-1 R.  Invariants: x1 * w = u and x2 * w = v (mod p), with w the word.
+  // Zero maps to zero.  Any 256-bit word is first reduced below p, so the
+  // loops always end (gcd(u, p) = 1 for 0 < u < p).
+  ZKP_HD Fp inv_binary() const {
+    const Fp p = from_raw(Params::P);
+    Fp u = *this;
+    while (!u.less_than(p)) u.sub_words(p);  // at most 5 times: 2^256 < 6p
+    if (u.is_zero()) return zero();
+    Fp v = p, x1 = zero(), x2 = zero();
+    x1.v[0] = 1;
+    while (!u.is_one_word() && !v.is_one_word()) {
+      while (!(u.v[0] & 1)) {
+        u.shr1();
+        x1.half_mod();
+      }
+      while (!(v.v[0] & 1)) {
+        v.shr1();
+        x2.half_mod();
+      }
+      if (!u.less_than(v)) {
+        u.sub_words(v);
+        x1 = x1 - x2;
+      } else {
+        v.sub_words(u);
+        x2 = x2 - x1;
+      }
+    }
+    return (u.is_one_word() ? x1 : x2) * from_raw(Params::R3);
+  }
+  ZKP_HD bool is_one_word() const {
+    u32 acc = v[0] ^ 1u;
+#pragma unroll
+    for (int i = 1; i < 8; i++) acc |= v[i];
+    return acc == 0;
+  }
+  ZKP_HD bool less_than(const Fp& o) const {
+    u32 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) (void)sbb(v[i], o.v[i], borrow);
+    return borrow != 0;
+  }
+  ZKP_HD void sub_words(const Fp& o) {
+    u32 borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = sbb(v[i], o.v[i], borrow);
+  }
+  ZKP_HD void shr1() {
+#pragma unroll
+    for (int i = 0; i < 7; i++) v[i] = (v[i] >> 1) | (v[i + 1] << 31);
+    v[7] >>= 1;
+  }
+  // x / 2 mod p for x < p: (x + p) / 2 when x is odd (x + p < 2p < 2^256)
+  ZKP_HD void half_mod() {
+    if (v[0] & 1) {
+      u32 carry = 0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = adc(v[i], Params::P[i], carry);
+    }
+    shr1();
   }
 
   // Fermat inverse x^(p-2); inverse of zero is zero.

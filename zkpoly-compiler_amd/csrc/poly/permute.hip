@@ -17,11 +17,12 @@
 //  1. canonical keys of A and S into two key buffers (one launch for both);
 //  2. sort, the same launches serving both arrays: every workgroup sorts a tile
 //     of 2048 keys in LDS (8 per thread by a sorting network, then merges of
-//     runs 8 .. 1024); merge passes then double the sorted runs through HBM,
-//     each workgroup producing 2048 outputs from a merge-path split staged in
-//     LDS.  Keys are whole 256-bit values compared by a borrow chain, so the
-//     cost does not depend on how the values are spread (small lookup tables,
-//     random-looking compressed expressions and negatives alike);
+//     runs 8 .. 1024); merge passes then double the sorted runs through HBM:
+//     one thread per output tile finds the tile's merge-path split, then each
+//     workgroup merges its 2048 outputs staged in LDS.  Keys are whole 256-bit
+//     values compared by a borrow chain, so the cost does not depend on how
+//     the values are spread (small lookup tables, random-looking compressed
+//     expressions and negatives alike);
 //  3. every run start of A' finds its lower bound in S' and marks that table
 //     slot used; A' is written out in Montgomery form;
 //  4. one scan over rows of (run starts, unused slots) counts compacts the
@@ -164,27 +165,60 @@ __global__ void __launch_bounds__(STH) k_tile_sort(Fr* k0, Fr* k1, uint32_t n) {
   for (uint32_t c = threadIdx.x; c < 2u * cnt; c += STH) dst[c] = s[swz(c)];
 }
 
-// 2b. Runs of L sorted keys (the last may be short) merged pairwise into runs
-// of 2L: workgroup b writes outputs [b*STILE, min((b+1)*STILE, n)).
-__global__ void __launch_bounds__(STH) k_merge_pass(const Fr* in0, Fr* out0, const Fr* in1, Fr* out1, uint32_t n,
-                                                     uint32_t L) {
-  __shared__ uint4 s[2 * STILE];
-  __shared__ uint32_t split[2];
+// The pair of runs of L (the last run may be short, or missing) holding
+// output position o: base pb, run lengths la and lb; B starts at pb + la
+// (la == L whenever lb > 0).
+struct Pair {
+  uint64_t pb;
+  uint32_t la, lb;
+};
+
+__device__ __forceinline__ Pair pair_of(uint64_t o, uint32_t n, uint32_t L) {
+  Pair p;
+  p.pb = o / (2ull * L) * (2ull * L);
+  const uint64_t rest = n - p.pb;
+  p.la = (uint32_t)(rest < L ? rest : L);
+  p.lb = (uint32_t)(rest > L ? (rest - L < L ? rest - L : L) : 0);
+  return p;
+}
+
+// 2b. split[y][b] = A's share of the outputs before b*STILE within its pair,
+// b < tiles: every search runs in its own thread, so no workgroup of the merge
+// waits on a chain of dependent loads.
+__global__ void __launch_bounds__(TH) k_merge_split(const Fr* in0, const Fr* in1, uint32_t n, uint32_t L,
+                                                    uint32_t tiles, uint32_t* split) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= tiles) return;
   const Fr* in = blockIdx.y ? in1 : in0;
-  Fr* out = pick(out0, out1);
-  const uint64_t o0 = (uint64_t)blockIdx.x * STILE;
-  const uint64_t o1 = o0 + STILE < n ? o0 + STILE : n;
-  const uint64_t pb = o0 / (2ull * L) * (2ull * L);
-  const uint32_t la = (uint32_t)(n - pb < L ? n - pb : L);
-  const uint32_t lb = (uint32_t)(n - pb > L ? (n - pb - L < L ? n - pb - L : L) : 0);
-  const Fr* A = in + pb;
-  const Fr* B = in + pb + la;  // la == L whenever lb > 0
+  const uint64_t o = (uint64_t)b * STILE;
+  const Pair p = pair_of(o, n, L);
+  const Fr* A = in + p.pb;
+  const Fr* B = A + p.la;
   const auto GA = [&](uint32_t i) { return Fr::load(A[i].v); };
   const auto GB = [&](uint32_t j) { return Fr::load(B[j].v); };
-  if (threadIdx.x < 2) split[threadIdx.x] = merge_path(GA, la, GB, lb, (uint32_t)((threadIdx.x ? o1 : o0) - pb));
-  __syncthreads();
+  split[blockIdx.y * tiles + b] = merge_path(GA, p.la, GB, p.lb, (uint32_t)(o - p.pb));
+}
+
+// 2c. Runs of L sorted keys merged pairwise into runs of 2L: workgroup b
+// writes outputs [b*STILE, min((b+1)*STILE, n)) from the splits of 2b.
+__global__ void __launch_bounds__(STH) k_merge_pass(const Fr* in0, Fr* out0, const Fr* in1, Fr* out1, uint32_t n,
+                                                     uint32_t L, const uint32_t* split) {
+  __shared__ uint4 s[2 * STILE];
+  const Fr* in = blockIdx.y ? in1 : in0;
+  Fr* out = pick(out0, out1);
+  const uint32_t tiles = gridDim.x;
+  const uint64_t o0 = (uint64_t)blockIdx.x * STILE;
+  const uint64_t o1 = o0 + STILE < n ? o0 + STILE : n;
+  const Pair p = pair_of(o0, n, L);
+  const uint64_t pb = p.pb;
+  const uint32_t la = p.la, lb = p.lb;
+  const Fr* A = in + pb;
+  const Fr* B = A + la;
   const uint32_t d0 = (uint32_t)(o0 - pb), d1 = (uint32_t)(o1 - pb);
-  const uint32_t i0 = split[0], i1 = split[1];
+  // the end of the tile is the end of its pair, or the start of the next tile
+  // inside the same pair
+  const uint32_t i0 = split[blockIdx.y * tiles + blockIdx.x];
+  const uint32_t i1 = d1 == la + lb ? la : split[blockIdx.y * tiles + blockIdx.x + 1];
   const uint32_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t na = i1 - i0, nb = j1 - j0;  // na + nb = o1 - o0 <= STILE
   const uint32_t tot = na + nb;
@@ -265,7 +299,7 @@ struct SinkCounts {
   Fr* leftover;
   uint32_t* rank;  // aliases `start`: element k is read before it is written
   PolyView res_table;
-  __device__ void operator()(uint64_t k, const CountOp::T& excl, const CountOp::T& e) const {
+  __device__ void operator()(uint64_t k, const CountOp::T& excl, const CountOp::T& e, const CountOp::T&) const {
     if (e.u) Fr::load(ss[k].v).store(leftover[excl.u].v);
     if (e.f) {
       st(res_table, k, Fr::from_canonical(Fr::load(sa[k].v)));
@@ -287,7 +321,7 @@ __global__ void __launch_bounds__(TH) k_fill_back(const uint32_t* rank, const Fr
 }
 
 struct Layout {
-  size_t keys, flags, part, total;
+  size_t keys, flags, part, split, total;
 };
 
 static Layout layout(uint64_t n) {
@@ -296,7 +330,8 @@ static Layout layout(uint64_t n) {
   l.keys = align256(m * sizeof(Fr));
   l.flags = align256(m * sizeof(uint32_t));
   l.part = align256(tiles_of(m) * sizeof(CountOp::T));
-  l.total = 4 * l.keys + 2 * l.flags + 2 * l.part;
+  l.split = align256(2 * ((m + STILE - 1) / STILE) * sizeof(uint32_t));
+  l.total = 4 * l.keys + 2 * l.flags + 2 * l.part + l.split;
   return l;
 }
 
@@ -337,6 +372,7 @@ hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long 
   uint32_t* start = reinterpret_cast<uint32_t*>(t + 4 * lay.keys + lay.flags);
   auto* partial = reinterpret_cast<CountOp::T*>(t + 4 * lay.keys + 2 * lay.flags);
   auto* carry = reinterpret_cast<CountOp::T*>(t + 4 * lay.keys + 2 * lay.flags + lay.part);
+  uint32_t* split = reinterpret_cast<uint32_t*>(t + 4 * lay.keys + 2 * lay.flags + 2 * lay.part);
 
   const unsigned g = grid_for(n);
   hipLaunchKernelGGL(k_to_keys, dim3(g, 2), dim3(TH), 0, stream, vi, vt, ka[0], ks[0], n);
@@ -344,8 +380,10 @@ hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long 
   hipLaunchKernelGGL(k_tile_sort, dim3(tiles, 2), dim3(STH), 0, stream, ka[0], ks[0], n);
   int cur = 0;
   for (uint64_t L = STILE; L < n; L <<= 1) {
+    hipLaunchKernelGGL(k_merge_split, dim3((tiles + TH - 1) / TH, 2), dim3(TH), 0, stream, ka[cur], ks[cur], n,
+                       (uint32_t)L, tiles, split);
     hipLaunchKernelGGL(k_merge_pass, dim3(tiles, 2), dim3(STH), 0, stream, ka[cur], ka[cur ^ 1], ks[cur],
-                       ks[cur ^ 1], n, (uint32_t)L);
+                       ks[cur ^ 1], n, (uint32_t)L, split);
     cur ^= 1;
   }
   const Fr* sa = ka[cur];

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(TH) k_fill(PolyView r, int mode) {
 }
 
 __global__ void k_inv_scalar(uint32_t* x) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) Fr::load(x).inv().store(x);
+  if (threadIdx.x == 0 && blockIdx.x == 0) Fr::load(x).inv_binary().store(x);
 }
 
 __global__ void k_pow_scalar(uint32_t* x, uint64_t e) {
@@ -113,8 +113,8 @@ struct SrcProd {  // x_k
 };
 struct SinkInclusive {  // x_k = prefix product through k
   PolyView v;
-  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T& e) const {
-    st(v, k, excl.v * e.v);
+  __device__ void operator()(uint64_t k, const ProdOp::T&, const ProdOp::T&, const ProdOp::T& incl) const {
+    st(v, k, incl.v);
   }
 };
 
@@ -134,13 +134,15 @@ struct SrcNonZeroRev {  // x_{n-1-k}, or 1 for zero
 };
 struct SinkStoreExcl {  // pre[k] = product of the non-zero x_j, j < k
   Fr* pre;
-  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&) const { pre[k] = excl.v; }
+  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&, const ProdOp::T&) const {
+    pre[k] = excl.v;
+  }
 };
 struct SinkInvert {  // x_i = pre[i] * suf[i] / total for x_i != 0 (i = n-1-k)
   PolyView v;
   const Fr* pre;
   const Fr* inv_total;
-  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&) const {
+  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&, const ProdOp::T&) const {
     const uint64_t i = v.len - 1 - k;
     const Fr x = ld(v, i);
     st(v, i, x.is_zero() ? Fr::zero() : pre[i] * excl.v * (*inv_total));
@@ -159,13 +161,13 @@ struct SrcHornerRev {  // (p_{n-1-k}, b)
 };
 struct SinkKate {  // q_{n-1-k} = S of the fold of (p_j, b), j > n-1-k
   PolyView q;
-  __device__ void operator()(uint64_t k, const HornerOp::T& excl, const HornerOp::T&) const {
+  __device__ void operator()(uint64_t k, const HornerOp::T& excl, const HornerOp::T&, const HornerOp::T&) const {
     st(q, q.len - 1 - k, excl.s);
   }
 };
 
 __global__ void k_invert_total(const Fr* total, Fr* inv) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *inv = total->inv();
+  if (threadIdx.x == 0 && blockIdx.x == 0) *inv = total->inv_binary();
 }
 
 __global__ void k_store_s(const HornerOp::T* total, uint32_t* res) {

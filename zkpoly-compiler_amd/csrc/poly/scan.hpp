@@ -3,8 +3,8 @@
 // rescan) used by the scans, folds and the permutation.
 //
 // A monoid Op supplies T, id() and comb(l, u) (l precedes u).  A source
-// src(k) yields element k as an Op::T; a sink(k, exclusive, element) receives
-// the fold of elements [0, k) and element k.  Every workgroup folds TILE =
+// src(k) yields element k as an Op::T; a sink(k, exclusive, element, inclusive)
+// receives the folds of elements [0, k) and [0, k] and element k.  Every workgroup folds TILE =
 // TH * PER consecutive elements (PER per thread, then an ordered LDS scan over
 // the TH thread folds).
 #pragma once
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(TH) k_partials_scan(const typename Op::T* __re
   if (total && threadIdx.x == 0) *total = tot;
 }
 
-// Rescan of every tile seeded with its carry; sink(k, exclusive, element)
+// Rescan of every tile seeded with its carry; sink(k, exclusive, element, inclusive)
 // receives the fold of elements [0, k) and element k.
 template <class Op, class Src, class Sink>
 __global__ void __launch_bounds__(TH)
@@ -108,8 +108,9 @@ __global__ void __launch_bounds__(TH)
     const uint64_t k = k0 + j;
     if (k >= n) break;
     const T e = src(k);
-    sink(k, acc, e);
-    acc = Op::comb(acc, e);
+    const T next = Op::comb(acc, e);
+    sink(k, acc, e, next);
+    acc = next;
   }
 }
 
