@@ -46,6 +46,27 @@ int zkp_fused_build(const char* const* graph_jsons, const char* const* names, in
                     const char* hipcc, const char* include_dir, const char* const* extra_flags, int n_extra,
                     char* err, size_t err_cap);
 
+/* library: libzkp_fused_pipeline.so */
+
+/* Streams a generated kernel over host-resident polynomials in divide_parts
+ * (> 3) chunks: three streams (copy in, compute, copy out), two device
+ * buffers per read-only polynomial and three per mutable one
+ * (dev_polys[j * num_polys + i], dev_mut_polys[j * num_mut_polys + i]), each
+ * of len / divide_parts elements; scalars in one-element device buffers,
+ * mutable scalars copied back at the end.  Argument arrays are laid out
+ * [scalars..., polys...] per buffer set in arg_buffer, which holds
+ * 2 * (num_scalars + num_polys) + 3 * (num_mut_scalars + num_mut_polys)
+ * PolyPtr.  Host polynomials are unsliced (rotation allowed).  Synchronous.
+ * Replaces PipelinedFusedKernel::get_fn, core/src/fused_kernels.rs:198-516
+ * (host code in the reference's runtime). */
+hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, const unsigned int* const* host_scalars,
+                               unsigned int* const* dev_scalars, unsigned int num_mut_scalars,
+                               unsigned int* const* host_mut_scalars, unsigned int* const* dev_mut_scalars,
+                               unsigned int num_polys, const ConstPolyPtr* host_polys, unsigned int* const* dev_polys,
+                               unsigned int num_mut_polys, const PolyPtr* host_mut_polys,
+                               unsigned int* const* dev_mut_polys, unsigned long long len, unsigned int divide_parts,
+                               void* arg_buffer, size_t arg_buffer_size, void* local_buffer, int device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,7 +50,7 @@ def exported(path):
 def test_headers_declare_something():
     secs = header_sections()
     assert {s[1] for s in secs} >= {"libntt.so", "libpoly.so", "libzkp_msm.so", "libmsm_bn254_16_16_false.so",
-                                    "libzkp_fused_codegen.so", "libzkp_serve.so"}
+                                    "libzkp_fused_codegen.so", "libzkp_fused_pipeline.so", "libzkp_serve.so"}
     assert all(s[2] for s in secs)
 
 
@@ -313,3 +313,27 @@ def test_permute_buffer_size_and_rejects(n):
     assert lib.permute(None, None, 60, whole, whole, part, part, None) == HIP_INVALID_VALUE
     empty = PolyPtr(0, 0, 0, 0, 0)
     assert lib.permute(0x9000, None, 0, whole, whole, empty, empty, None) == 0
+
+
+def test_fused_pipelined_rejects_before_any_work():
+    """The reference's asserts (core/src/fused_kernels.rs:202, 229-242, 262,
+    318-321, 335) become hipErrorInvalidValue returned before a stream exists."""
+    from zkpoly_amd.fused import _pipeline_lib
+
+    f = _pipeline_lib(Libs()).zkp_fused_pipelined
+    host = (PolyPtr * 1)(PolyPtr(0x1000, 64, 3, 0, 64))
+    dev = (ctypes.c_void_p * 3)(0x2000, 0x3000, 0x4000)
+    none = (ctypes.c_void_p * 1)()
+
+    def call(host_p=host, len_=64, parts=4, argsz=3 * 40, kernel=0x5000, arg=0x6000):
+        return f(kernel, 0, none, none, 0, none, none, 0, None, none, 1, host_p, dev, len_, parts, arg, argsz, None, 0)
+
+    assert call(parts=3) == HIP_INVALID_VALUE  # divide_parts must exceed 3
+    assert call(len_=66) == HIP_INVALID_VALUE  # len a multiple of divide_parts
+    assert call(argsz=3 * 40 - 1) == HIP_INVALID_VALUE  # three sets of one mut var
+    assert call(kernel=0) == HIP_INVALID_VALUE
+    assert call(arg=0) == HIP_INVALID_VALUE
+    sliced = (PolyPtr * 1)(PolyPtr(0x1000, 64, 0, 8, 128))
+    assert call(host_p=sliced) == HIP_INVALID_VALUE  # "pipelined fused kernel doesn't support slice"
+    short = (PolyPtr * 1)(PolyPtr(0x1000, 32, 0, 0, 32))
+    assert call(host_p=short) == HIP_INVALID_VALUE

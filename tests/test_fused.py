@@ -222,3 +222,114 @@ def test_batched_entry_matches_single(libs, stream):
     for (ob, w), n in zip(wants, lens):
         if n:
             assert np.array_equal(ob.download(np.uint64, n * 4).reshape(n, 4), w)
+
+
+# ---------------------------------------------------------------- pipelined form
+
+
+class HostPoly:
+    """A pinned host polynomial, optionally rotated (logical i at physical (i - r) mod n)."""
+
+    def __init__(self, data, rotate=0):
+        n = data.shape[0]
+        self.buf = hip.PinnedBuffer(n * 32)
+        self.arr = ScalarArray(n, self.buf.addr, device="cpu")
+        self.arr.rotate(rotate)
+        view = self.buf.view().reshape(n, 4)
+        idx = (np.arange(n) + n - self.arr.get_rotation()) % n  # ScalarArray.physical_index, vectorised
+        assert idx[n - 1] == self.arr.physical_index(n - 1)
+        view[idx] = data
+        self.idx = idx
+
+    def logical(self):
+        return self.buf.view().reshape(-1, 4)[self.idx].copy()
+
+
+class HostScalar:
+    def __init__(self, value):
+        self.buf = hip.PinnedBuffer(32)
+        self.buf.view()[:] = value
+        self.s = Scalar(self.buf.addr, device="cpu")
+
+    def value(self):
+        return self.buf.view().copy()
+
+
+def dev_buffers(count, nbytes):
+    bufs = [hip.DeviceBuffer(nbytes) for _ in range(count)]
+    return bufs, [GpuBuffer(b.addr, nbytes) for b in bufs]
+
+
+def pipelined(name, nv, nm, parts, scalars, mut_scalars):
+    m = fused.FusedKernelMeta(name, nv, nm, fused.DEMO_DIR, fused.PipelinedMeta(parts, scalars, mut_scalars))
+    return fused.PipelinedFusedKernel(Libs(), m)
+
+
+def run_pipelined(k, parts, scalars, polys, mut_scalars, mut_polys, n):
+    """Lays out the reference's argument lists (core/src/fused_kernels.rs:219-227)."""
+    chunk_bytes = (n // parts) * 32
+    arg_b, (arg,) = dev_buffers(1, fused.pipelined_arg_buffer_bytes(k.meta))
+    loc_b, (loc,) = dev_buffers(1, 256)
+    ms_b, ms_g = dev_buffers(len(mut_scalars), 32)
+    mp_b, mp_g = dev_buffers(3 * len(mut_polys), chunk_bytes)
+    s_b, s_g = dev_buffers(len(scalars), 32)
+    p_b, p_g = dev_buffers(2 * len(polys), chunk_bytes)
+    mut_var = [arg, loc] + [s.s for s in mut_scalars] + [p.arr for p in mut_polys] + ms_g + mp_g
+    var = [s.s for s in scalars] + [p.arr for p in polys] + s_g + p_g
+    k.get_fn().f(mut_var, var, lambda i: i)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,parts", [(1 << 20, 8), (4 * 3000, 4), (1 << 16, 32)])
+def test_pipelined_ab_gamma_c_minus_z(stream, n, parts):
+    a, b, c = (random_elements(n, 300 + i) for i in range(3))
+    gamma, z = random_elements(1, 310)[0], random_elements(1, 311)[0]
+    ha, hb, hc = HostPoly(a, 1000), HostPoly(b), HostPoly(c, n - 7)
+    out = HostPoly(np.zeros((n, 4), np.uint64), 5)
+    k = pipelined("fused_arith_pipe_abgcz", 5, 1, parts, 2, 0)
+    run_pipelined(k, parts, [HostScalar(gamma), HostScalar(z)], [ha, hb, hc], [], [out], n)
+    g = np.broadcast_to(gamma, (n, 4)).copy()
+    zz = np.broadcast_to(z, (n, 4)).copy()
+    want = orc.vec("orc_vec_sub", FR, orc.vec("orc_vec_add", FR, orc.vec("orc_vec_mul", FR, a, b),
+                                              orc.vec("orc_vec_mul", FR, g, c)), zz)
+    assert np.array_equal(out.logical(), want)
+
+
+@pytest.mark.gpu
+def test_pipelined_mutable_scalar_and_in_place(libs, stream):
+    n, parts = 6 * 4096, 6
+    x, y = random_elements(n, 320), random_elements(n, 321)
+    s = random_elements(1, 322)[0]
+    hx, hy = HostPoly(x, 77), HostPoly(y, 4096 * 5 + 3)
+    hq = HostPoly(np.zeros((n, 4), np.uint64))
+    ht = HostScalar(np.zeros(4, np.uint64))
+    k = pipelined("fused_arith_pipe_mixed", 2, 3, parts, 1, 1)
+    run_pipelined(k, parts, [HostScalar(s)], [hy], [ht], [hx, hq], n)
+    t = orc.add(FR, orc.mul(FR, s, s), s)
+    assert np.array_equal(ht.value(), t)
+    tv = np.broadcast_to(t, (n, 4)).copy()
+    assert np.array_equal(hx.logical(), orc.vec("orc_vec_mul", FR, tv, orc.vec("orc_vec_add", FR, x, y)))
+    assert np.array_equal(hq.logical(), orc.vec("orc_vec_neg", FR, orc.vec("orc_vec_sub", FR, x, y)))
+
+
+@pytest.mark.gpu
+def test_pipelined_coefficient_rules_see_element_zero_once(stream):
+    """Only the first chunk carries is_first, so the constant-coefficient rules
+    touch global element 0 and nothing at the start of later chunks."""
+    n, parts = 4 * 1024, 4
+    p = random_elements(n, 330)
+    s = random_elements(1, 331)[0]
+    outs = [HostPoly(np.zeros((n, 4), np.uint64), r) for r in (0, 1, 1024, n - 1)]
+    k = pipelined("fused_arith_pipe_coef", 2, 4, parts, 1, 0)
+    run_pipelined(k, parts, [HostScalar(s)], [HostPoly(p, 9)], [], outs, n)
+    sv = np.broadcast_to(s, (n, 4)).copy()
+    neg_p = orc.vec("orc_vec_neg", FR, p)
+    w0 = p.copy()
+    w0[0] = orc.add(FR, s, p[0])
+    w1 = neg_p.copy()
+    w1[0] = orc.sub(FR, s, p[0])
+    w2 = p.copy()
+    w2[0] = orc.sub(FR, p[0], s)
+    w3 = orc.vec("orc_vec_mul", FR, sv, p)
+    for i, w in enumerate((w0, w1, w2, w3)):
+        assert np.array_equal(outs[i].logical(), w), i

@@ -1,0 +1,229 @@
+// libzkp_fused_pipeline.so -- host-chunked streaming of a fused kernel over
+// host-resident polynomials.
+//
+// The native driver behind PipelinedFusedKernel (the reference's closure in
+// core/src/fused_kernels.rs:198-516, meta runtime/src/functions.rs:37-50).
+// The polynomials live in host memory; the kernel runs on divide_parts chunks
+// of chunk_len = len / divide_parts elements each, in caller-provided device
+// buffers: two per read-only polynomial (one loading, one computing) and three
+// per mutable polynomial (loading, computing, storing back).  Three streams of
+// default priority carry the copies in, the kernels and the copies out;
+// events order them exactly as the reference does:
+//   copy-in of mutable chunk c waits for copy-out of the chunk that last used
+//   its buffer (c - 3); copy-in of read-only chunk c waits for the kernel that
+//   last used its buffer (c - 2); the kernel waits for both copy-ins; the
+//   copy-out waits for the kernel.
+// Scalars travel once: read-only and mutable scalars in before the first
+// chunk, mutable scalars back after the last.  The kernel sees is_first only
+// on chunk 0 (coefficient-form rules index the global element 0).  The call is
+// synchronous: it returns after the last copy-out, like the reference.
+//
+// Host polynomials may be rotated (the reference refuses slices, and so does
+// this driver); a chunk whose physical range wraps is copied in two pieces.
+// For copies that overlap the kernels the host memory must be pinned
+// (hipHostMalloc); pageable memory still gives the right result.
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+extern "C" {
+typedef struct PolyPtr {
+  uint32_t* ptr;
+  size_t len;
+  size_t rotate;
+  size_t offset;
+  size_t whole_len;
+} PolyPtr;
+typedef struct ConstPolyPtr {
+  const uint32_t* ptr;
+  size_t len;
+  size_t rotate;
+  size_t offset;
+  size_t whole_len;
+} ConstPolyPtr;
+typedef hipError_t (*zkp_fused_fn)(const ConstPolyPtr* vars, const PolyPtr* mut_vars, unsigned long long len,
+                                   bool is_first, void* local_buffer, hipStream_t stream);
+}
+
+namespace {
+
+constexpr size_t ELEM = 32;
+
+struct Err {
+  hipError_t first = hipSuccess;
+  void operator()(hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  }
+};
+
+// Copies logical elements [start, start + count) of a rotated host array of
+// `len` elements (logical i at physical (i + len - rotate) % len) to `dev`.
+void copy_in(Err& err, const uint32_t* host, size_t len, size_t rotate, size_t start, size_t count, uint32_t* dev,
+             hipStream_t s) {
+  size_t phys = (start + len - rotate) % len;
+  const size_t first = count < len - phys ? count : len - phys;
+  err(hipMemcpyAsync(dev, host + phys * 8, first * ELEM, hipMemcpyHostToDevice, s));
+  if (first < count) err(hipMemcpyAsync(dev + first * 8, host, (count - first) * ELEM, hipMemcpyHostToDevice, s));
+}
+
+void copy_out(Err& err, uint32_t* host, size_t len, size_t rotate, size_t start, size_t count, const uint32_t* dev,
+              hipStream_t s) {
+  size_t phys = (start + len - rotate) % len;
+  const size_t first = count < len - phys ? count : len - phys;
+  err(hipMemcpyAsync(host + phys * 8, dev, first * ELEM, hipMemcpyDeviceToHost, s));
+  if (first < count) err(hipMemcpyAsync(host, dev + first * 8, (count - first) * ELEM, hipMemcpyDeviceToHost, s));
+}
+
+// A host polynomial accepted by the driver: unsliced, `len` elements.
+bool plain(const uint32_t* ptr, size_t len, size_t rotate, size_t offset, size_t whole_len, size_t want) {
+  return ptr != nullptr && len == want && offset == 0 && (whole_len == 0 || whole_len == len) &&
+         (rotate == 0 || rotate < len);
+}
+
+// Events that may be waited on before their first record are skipped until
+// then (a wait on a never-recorded event is a no-op in the reference's CUDA).
+struct Ev {
+  hipEvent_t e = nullptr;
+  bool recorded = false;
+};
+
+}  // namespace
+
+extern "C" {
+
+// Runs `kernel` over host polynomials in divide_parts chunks (see above).
+// Device buffers: dev_polys[j * num_polys + i] (j < 2) and
+// dev_mut_polys[j * num_mut_polys + i] (j < 3) of chunk_len elements each;
+// dev_scalars / dev_mut_scalars of one element each.  arg_buffer holds
+// (2 * (num_scalars + num_polys) + 3 * (num_mut_scalars + num_mut_polys))
+// PolyPtr; local_buffer is passed to the kernel.  hipErrorInvalidValue, before
+// any work, when a precondition of the reference fails.
+hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, const uint32_t* const* host_scalars,
+                               uint32_t* const* dev_scalars, unsigned int num_mut_scalars,
+                               uint32_t* const* host_mut_scalars, uint32_t* const* dev_mut_scalars,
+                               unsigned int num_polys, const ConstPolyPtr* host_polys, uint32_t* const* dev_polys,
+                               unsigned int num_mut_polys, const PolyPtr* host_mut_polys,
+                               uint32_t* const* dev_mut_polys, unsigned long long len, unsigned int divide_parts,
+                               void* arg_buffer, size_t arg_buffer_size, void* local_buffer, int device) {
+  const unsigned nv = num_scalars + num_polys, nm = num_mut_scalars + num_mut_polys;
+  if (kernel == nullptr || arg_buffer == nullptr || divide_parts <= 3 || len == 0 || len % divide_parts != 0)
+    return hipErrorInvalidValue;
+  if (arg_buffer_size < (size_t)(2 * nv + 3 * nm) * sizeof(PolyPtr)) return hipErrorInvalidValue;
+  for (unsigned i = 0; i < num_scalars; i++)
+    if (!host_scalars[i] || !dev_scalars[i]) return hipErrorInvalidValue;
+  for (unsigned i = 0; i < num_mut_scalars; i++)
+    if (!host_mut_scalars[i] || !dev_mut_scalars[i]) return hipErrorInvalidValue;
+  for (unsigned i = 0; i < num_polys; i++) {
+    const ConstPolyPtr& p = host_polys[i];
+    if (!plain(p.ptr, p.len, p.rotate, p.offset, p.whole_len, len)) return hipErrorInvalidValue;
+    for (unsigned j = 0; j < 2; j++)
+      if (!dev_polys[j * num_polys + i]) return hipErrorInvalidValue;
+  }
+  for (unsigned i = 0; i < num_mut_polys; i++) {
+    const PolyPtr& p = host_mut_polys[i];
+    if (!plain(p.ptr, p.len, p.rotate, p.offset, p.whole_len, len)) return hipErrorInvalidValue;
+    for (unsigned j = 0; j < 3; j++)
+      if (!dev_mut_polys[j * num_mut_polys + i]) return hipErrorInvalidValue;
+  }
+  const size_t chunk = len / divide_parts;
+
+  Err err;
+  err(hipSetDevice(device));
+  if (err.first != hipSuccess) return err.first;
+  hipStream_t h2d = nullptr, comp = nullptr, d2h = nullptr;
+  err(hipStreamCreate(&h2d));
+  err(hipStreamCreate(&comp));
+  err(hipStreamCreate(&d2h));
+  Ev mut_h2d[3], mut_comp[3], mut_d2h[3], h2d_done[2], comp_done[2];
+  for (Ev* set : {mut_h2d, mut_comp, mut_d2h})
+    for (int j = 0; j < 3; j++) err(hipEventCreateWithFlags(&set[j].e, hipEventDisableTiming));
+  for (Ev* set : {h2d_done, comp_done})
+    for (int j = 0; j < 2; j++) err(hipEventCreateWithFlags(&set[j].e, hipEventDisableTiming));
+
+  // argument arrays: [scalars..., polys...] per buffer set, as the reference
+  // lays them out (fused_kernels.rs:358-408); the host copy stays alive until
+  // the final synchronisation
+  std::vector<ConstPolyPtr> vars(2 * (size_t)nv);
+  std::vector<PolyPtr> muts(3 * (size_t)nm);
+  for (unsigned j = 0; j < 2; j++) {
+    for (unsigned i = 0; i < num_scalars; i++) vars[j * nv + i] = ConstPolyPtr{dev_scalars[i], 1, 0, 0, 1};
+    for (unsigned i = 0; i < num_polys; i++)
+      vars[j * nv + num_scalars + i] = ConstPolyPtr{dev_polys[j * num_polys + i], chunk, 0, 0, chunk};
+  }
+  for (unsigned j = 0; j < 3; j++) {
+    for (unsigned i = 0; i < num_mut_scalars; i++) muts[j * nm + i] = PolyPtr{dev_mut_scalars[i], 1, 0, 0, 1};
+    for (unsigned i = 0; i < num_mut_polys; i++)
+      muts[j * nm + num_mut_scalars + i] = PolyPtr{dev_mut_polys[j * num_mut_polys + i], chunk, 0, 0, chunk};
+  }
+  ConstPolyPtr* d_vars = reinterpret_cast<ConstPolyPtr*>(arg_buffer);
+  PolyPtr* d_muts = reinterpret_cast<PolyPtr*>(d_vars + 2 * nv);
+  if (err.first == hipSuccess) {
+    if (nv) err(hipMemcpyAsync(d_vars, vars.data(), vars.size() * sizeof(ConstPolyPtr), hipMemcpyHostToDevice, h2d));
+    if (nm) err(hipMemcpyAsync(d_muts, muts.data(), muts.size() * sizeof(PolyPtr), hipMemcpyHostToDevice, h2d));
+    for (unsigned i = 0; i < num_mut_scalars; i++)
+      err(hipMemcpyAsync(dev_mut_scalars[i], host_mut_scalars[i], ELEM, hipMemcpyHostToDevice, h2d));
+    for (unsigned i = 0; i < num_scalars; i++)
+      err(hipMemcpyAsync(dev_scalars[i], host_scalars[i], ELEM, hipMemcpyHostToDevice, h2d));
+  }
+
+  auto wait = [&](hipStream_t s, Ev& ev) {
+    if (ev.recorded) err(hipStreamWaitEvent(s, ev.e, 0));
+  };
+  auto record = [&](Ev& ev, hipStream_t s) {
+    err(hipEventRecord(ev.e, s));
+    ev.recorded = true;
+  };
+
+  for (unsigned c = 0, mb = 0, b = 0; c < divide_parts && err.first == hipSuccess; c++) {
+    const size_t start = (size_t)c * chunk;
+    // mutable chunk in, once its buffer's previous copy-out is done
+    wait(h2d, mut_d2h[mb]);
+    for (unsigned i = 0; i < num_mut_polys; i++) {
+      const PolyPtr& p = host_mut_polys[i];
+      copy_in(err, p.ptr, len, p.rotate, start, chunk, dev_mut_polys[mb * num_mut_polys + i], h2d);
+    }
+    record(mut_h2d[mb], h2d);
+    // read-only chunk in, once the kernel that last read its buffer is done
+    wait(h2d, comp_done[b]);
+    for (unsigned i = 0; i < num_polys; i++) {
+      const ConstPolyPtr& p = host_polys[i];
+      copy_in(err, p.ptr, len, p.rotate, start, chunk, dev_polys[b * num_polys + i], h2d);
+    }
+    record(h2d_done[b], h2d);
+    // the kernel on this chunk
+    wait(comp, mut_h2d[mb]);
+    wait(comp, h2d_done[b]);
+    if (err.first == hipSuccess)
+      err(kernel(d_vars + (size_t)b * nv, d_muts + (size_t)mb * nm, chunk, c == 0, local_buffer, comp));
+    record(comp_done[b], comp);
+    record(mut_comp[mb], comp);
+    // mutable chunk back
+    wait(d2h, mut_comp[mb]);
+    for (unsigned i = 0; i < num_mut_polys; i++) {
+      const PolyPtr& p = host_mut_polys[i];
+      copy_out(err, p.ptr, len, p.rotate, start, chunk, dev_mut_polys[mb * num_mut_polys + i], d2h);
+    }
+    record(mut_d2h[mb], d2h);
+    mb = (mb + 1) % 3;
+    b = (b + 1) % 2;
+  }
+  for (unsigned i = 0; i < num_mut_scalars; i++)
+    err(hipMemcpyAsync(host_mut_scalars[i], dev_mut_scalars[i], ELEM, hipMemcpyDeviceToHost, d2h));
+
+  // drain and release, whatever happened above
+  for (hipStream_t s : {h2d, comp, d2h})
+    if (s) err(hipStreamSynchronize(s));
+  for (Ev* set : {mut_h2d, mut_comp, mut_d2h})
+    for (int j = 0; j < 3; j++)
+      if (set[j].e) err(hipEventDestroy(set[j].e));
+  for (Ev* set : {h2d_done, comp_done})
+    for (int j = 0; j < 2; j++)
+      if (set[j].e) err(hipEventDestroy(set[j].e));
+  for (hipStream_t s : {h2d, comp, d2h})
+    if (s) err(hipStreamDestroy(s));
+  return err.first;
+}
+
+}  // extern "C"

@@ -136,6 +136,18 @@ def build_fused_demo() -> str:
     return fused.build_library(Libs(), fused.demo_kernels(), fused.DEMO_DIR)
 
 
+def build_fused_pipeline() -> str:
+    """Host-only driver of the pipelined fused kernel (HIP runtime API only)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libzkp_fused_pipeline.so")
+    tmp = out + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run([shutil.which("g++") or "g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          f"-I{rocm}/include", os.path.join(CSRC, "fused", "pipeline.cpp"), f"-L{rocm}/lib", "-lamdhip64", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_serve() -> str:
     """Host-only serving engine; links libntt.so and libzkp_msm.so."""
     os.makedirs(LIB, exist_ok=True)
@@ -157,7 +169,7 @@ def build_intpeak() -> str:
 
 
 def targets() -> list:
-    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_intpeak]
+    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_fused_pipeline, build_intpeak]
 
 
 def build_all(jobs: int | None = None) -> list[str]:

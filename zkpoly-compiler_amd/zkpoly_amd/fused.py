@@ -11,6 +11,10 @@ Mirrors (reference paths):
     [stream, inputs...], mut_vars = [arg_buffer, local_buffer, outputs...];
     the PolyPtr arrays are copied into arg_buffer and the kernel is called
     with is_first = true.
+  * PipelinedFusedKernel::get_fn (core/src/fused_kernels.rs:198-516): the same
+    kernel streamed over host-resident polynomials in divide_parts chunks by
+    the native driver in libzkp_fused_pipeline.so (three streams, two device
+    buffers per read-only polynomial, three per mutable one).
 GraphBuilder writes the serde JSON of common::arith::ArithGraph
 (common/src/arith.rs:172-331) so graphs can be made without the compiler.
 """
@@ -138,11 +142,50 @@ def graph_inplace_mixed() -> str:
     return g.to_json()
 
 
+def graph_pipelined_abgcz() -> str:
+    """a*b + gamma*c - z with the pipelined argument order, scalars first
+    (core/src/fused_kernels.rs:358-386): vars [gamma, z, a, b, c]; mut_vars [out]."""
+    g = GraphBuilder("Lagrange")
+    gamma, z = g.input(scalar=True), g.input(scalar=True)
+    a, b, c = g.input(), g.input(), g.input()
+    s = g.bin("Pp", "Add", g.bin("Pp", "Mul", a, b), g.bin("Sp", "Mul", gamma, c))
+    g.output(g.bin("Sp", "SubBy", z, s))
+    return g.to_json()
+
+
+def graph_pipelined_mixed() -> str:
+    """Pipelined order with a mutable scalar and an in-place polynomial:
+    vars [s, y]; mut_vars [t, x', q] with t = s*s + s (scalar), x' = (x + y) * t
+    (overwrites x), q = -(x - y)."""
+    g = GraphBuilder("Lagrange")
+    s = g.input(scalar=True)
+    y = g.input()
+    x = g.input(mutable=True)
+    t = g.bin("Ss", "Add", g.bin("Ss", "Mul", s, s), s)
+    g.output(t, scalar=True)
+    g.output(g.bin("Sp", "Mul", t, g.bin("Pp", "Add", x, y)), in_node=x)
+    g.output(g.neg(g.bin("Pp", "Sub", x, y)))
+    return g.to_json()
+
+
+def graph_pipelined_coef() -> str:
+    """Coefficient-form rules in pipelined order: vars [s, p]; mut_vars
+    [s + p, s - p, p - s, s * p]; element 0 exists once, in the first chunk."""
+    g = GraphBuilder("Coef")
+    s, p = g.input(scalar=True), g.input()
+    for op in ("Add", "Sub", "SubBy", "Mul"):
+        g.output(g.bin("Sp", op, s, p))
+    return g.to_json()
+
+
 def demo_kernels() -> list[tuple[str, str]]:
     return [
         (graph_ab_gamma_c_minus_z(), "fused_arith_abgcz"),
         (graph_coef_rules(), "fused_arith_coef_rules"),
         (graph_inplace_mixed(), "fused_arith_inplace_mixed"),
+        (graph_pipelined_abgcz(), "fused_arith_pipe_abgcz"),
+        (graph_pipelined_mixed(), "fused_arith_pipe_mixed"),
+        (graph_pipelined_coef(), "fused_arith_pipe_coef"),
     ]
 
 
@@ -195,11 +238,22 @@ def build_library(libs: Libs, kernels: list[tuple[str, str]], out_dir: str) -> s
 
 
 @dataclass
+class PipelinedMeta:
+    """runtime/src/functions.rs:45-50: how many chunks (> 3; lengths must be
+    multiples of it) and how many of the vars / mut_vars are scalars."""
+
+    divide_parts: int
+    num_scalars: int
+    num_mut_scalars: int
+
+
+@dataclass
 class FusedKernelMeta:
     name: str
     num_vars: int
     num_mut_vars: int
     lib_path: str
+    pipelined_meta: Optional[PipelinedMeta] = None
 
 
 def arg_buffer_bytes(meta: FusedKernelMeta) -> int:
@@ -278,3 +332,106 @@ class FusedKernel(RegisteredFunction):
             stream.sync()
 
         return Function(FuncMeta(meta.name, KernelType.FusedArith, meta), f)
+
+
+PIPELINE_LIB = "libzkp_fused_pipeline.so"
+
+
+def _pipeline_lib(libs: Libs) -> ctypes.CDLL:
+    lib = libs.load_relative(PIPELINE_LIB)
+    if not getattr(lib, "_zkp_typed", False):
+        vp, u = ctypes.c_void_p, ctypes.c_uint
+        lib.zkp_fused_pipelined.argtypes = [vp, u, vp, vp, u, vp, vp, u, vp, vp, u, vp, vp, ctypes.c_ulonglong, u,
+                                            vp, ctypes.c_size_t, vp, ctypes.c_int]
+        lib.zkp_fused_pipelined.restype = ctypes.c_int
+        lib._zkp_typed = True
+    return lib
+
+
+def pipelined_arg_buffer_bytes(meta: FusedKernelMeta) -> int:
+    """Argument buffer of the pipelined form: two sets of vars, three of
+    mut_vars (core/src/fused_kernels.rs:238-241, temporary_space.rs:58-97)."""
+    return (2 * meta.num_vars + 3 * meta.num_mut_vars) * POLYPTR_BYTES
+
+
+class PipelinedFusedKernel(RegisteredFunction):
+    """core/src/fused_kernels.rs:198-516.  With p mut scalars, n mut polys,
+    q scalars and m polys (all host-resident, polys unsliced):
+      mut_var = [arg_buffer, local_buffer, p mut scalars, n mut polys,
+                 p scalar GpuBuffers, 3n poly GpuBuffers (set j: [j*n, (j+1)*n))]
+      var     = [q scalars, m polys, q scalar GpuBuffers, 2m poly GpuBuffers]
+    Poly buffers hold len / divide_parts elements; scalar buffers one."""
+
+    def __init__(self, libs: Libs, meta: FusedKernelMeta):
+        assert meta.pipelined_meta is not None
+        assert meta.pipelined_meta.divide_parts > 3
+        self.kernel = FusedKernel(libs, meta)
+        self.meta = meta
+        self.driver = _pipeline_lib(libs).zkp_fused_pipelined
+
+    def get_fn(self) -> Function:
+        meta, driver = self.meta, self.driver
+        pm = meta.pipelined_meta
+        kernel_addr = ctypes.cast(self.kernel.c_func, ctypes.c_void_p).value
+        p, q, parts = pm.num_mut_scalars, pm.num_scalars, pm.divide_parts
+
+        def f(mut_var, var, gpu_mapping=None):
+            arg_buffer, local_buffer = mut_var[0], mut_var[1]
+            assert isinstance(arg_buffer, GpuBuffer) and isinstance(local_buffer, GpuBuffer)
+            assert arg_buffer.size >= pipelined_arg_buffer_bytes(meta)
+            rest = mut_var[2:]
+            assert (len(rest) - 2 * p) % 4 == 0
+            assert (len(var) - 2 * q) % 3 == 0
+            n = (len(rest) - 2 * p) // 4
+            m = (len(var) - 2 * q) // 3
+            assert n + p == meta.num_mut_vars
+            assert m + q == meta.num_vars
+            mut_scalars = [_host_scalar(v) for v in rest[:p]]
+            mut_polys = [_host_poly(v) for v in rest[p:p + n]]
+            mut_scalar_bufs = [_scalar_buffer(v) for v in rest[p + n:2 * p + n]]
+            mut_poly_bufs = list(rest[2 * p + n:])
+            scalars = [_host_scalar(v) for v in var[:q]]
+            polys = [_host_poly(v) for v in var[q:q + m]]
+            scalar_bufs = [_scalar_buffer(v) for v in var[q + m:2 * q + m]]
+            poly_bufs = list(var[2 * q + m:])
+            length = mut_polys[0].len if n else polys[0].len if m else 1
+            assert length % parts == 0
+            chunk = length // parts
+            for b in mut_poly_bufs + poly_bufs:
+                assert isinstance(b, GpuBuffer) and b.size == chunk * 32
+            for a in mut_polys + polys:
+                assert a.len == length
+
+            def arr(ctype, items):
+                return (ctype * max(1, len(items)))(*items)
+
+            c_vp = ctypes.c_void_p
+            hs = arr(c_vp, [s.value for s in scalars])
+            ds = arr(c_vp, [b.ptr for b in scalar_bufs])
+            hms = arr(c_vp, [s.value for s in mut_scalars])
+            dms = arr(c_vp, [b.ptr for b in mut_scalar_bufs])
+            hp = arr(PolyPtr, [PolyPtr.from_array(a) for a in polys])
+            dp = arr(c_vp, [b.ptr for b in poly_bufs])
+            hmp = arr(PolyPtr, [PolyPtr.from_array(a) for a in mut_polys])
+            dmp = arr(c_vp, [b.ptr for b in mut_poly_bufs])
+            device = gpu_mapping(0) if gpu_mapping is not None else 0
+            hip.check(driver(kernel_addr, q, hs, ds, p, hms, dms, m, hp, dp, n, hmp, dmp, length, parts,
+                             arg_buffer.ptr, arg_buffer.size, local_buffer.ptr, device), meta.name + " (pipelined)")
+
+        return Function(FuncMeta(meta.name, KernelType.FusedArith, meta), f)
+
+
+def _host_scalar(v) -> Scalar:
+    assert isinstance(v, Scalar) and v.device == "cpu", "pipelined scalars live in host memory"
+    return v
+
+
+def _host_poly(v) -> ScalarArray:
+    assert isinstance(v, ScalarArray) and v.device == "cpu", "pipelined polynomials live in host memory"
+    assert v.slice_info is None, "pipelined fused kernel doesn't support slice"
+    return v
+
+
+def _scalar_buffer(v) -> GpuBuffer:
+    assert isinstance(v, GpuBuffer) and v.size == 32
+    return v
