@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--serve-count", type=int, default=60, help="requests per rate in the serving leg")
     ap.add_argument("--msm-log-n", type=int, default=20)
     ap.add_argument("--poly-only", action="store_true", help="libpoly.so leg only")
+    ap.add_argument("--pipeline-only", action="store_true", help="pipelined fused-kernel leg only")
     return ap.parse_args()
 
 
@@ -473,6 +474,102 @@ def poly_leg(libs, stream, log_n, steps, with_cpu):
     return out
 
 
+def pipeline_leg(libs, stream, log_n, parts, steps):
+    """PipelinedFusedKernel: out = a*b + gamma*c - z over 2^log_n host-resident
+    (pinned) elements in `parts` chunks, against one whole copy-in, kernel and
+    copy-out on a single stream, and the plain pinned copy rates of the link."""
+    from zkpoly_amd import fused, hip
+    from zkpoly_amd.field import random_elements
+    from zkpoly_amd.runtime import GpuBuffer, PolyPtr, Scalar, ScalarArray
+
+    n = 1 << log_n
+    nbytes = n * 32
+    host = [hip.PinnedBuffer(nbytes) for _ in range(4)]  # a, b, c, out
+    for i in range(3):
+        host[i].view()[:] = random_elements(n, 500 + i).reshape(-1)
+    sc = [hip.PinnedBuffer(32) for _ in range(2)]
+    for i in range(2):
+        sc[i].view()[:] = random_elements(1, 510 + i)[0]
+    meta = fused.FusedKernelMeta("fused_arith_pipe_abgcz", 5, 1, fused.DEMO_DIR, fused.PipelinedMeta(parts, 2, 0))
+    k = fused.PipelinedFusedKernel(libs, meta)
+    chunk = nbytes // parts
+    keep = []
+
+    def dev(nb):
+        b = hip.DeviceBuffer(nb)
+        keep.append(b)
+        return GpuBuffer(b.addr, nb)
+
+    arg, loc = dev(fused.pipelined_arg_buffer_bytes(meta)), dev(256)
+    mut_var = [arg, loc, ScalarArray(n, host[3].addr, device="cpu")] + [dev(chunk) for _ in range(3)]
+    var = ([Scalar(sc[0].addr, device="cpu"), Scalar(sc[1].addr, device="cpu")]
+           + [ScalarArray(n, host[i].addr, device="cpu") for i in range(3)]
+           + [dev(32), dev(32)] + [dev(chunk) for _ in range(6)])
+    f = k.get_fn().f
+    f(mut_var, var, lambda i: i)  # warm
+    t = time.perf_counter()
+    for _ in range(steps):
+        f(mut_var, var, lambda i: i)
+    piped_ms = (time.perf_counter() - t) * 1e3 / steps
+
+    # whole-array reference point: copy in, one kernel, copy out, one stream
+    whole = [hip.DeviceBuffer(nbytes) for _ in range(4)]
+    dsc = [hip.DeviceBuffer(32) for _ in range(2)]
+    plain = fused.FusedKernel(libs, fused.FusedKernelMeta("fused_arith_pipe_abgcz", 5, 1, fused.DEMO_DIR))
+    args = (PolyPtr * 6)(PolyPtr(dsc[0].addr, 1, 0, 0, 1), PolyPtr(dsc[1].addr, 1, 0, 0, 1),
+                         *[PolyPtr(whole[i].addr, n, 0, 0, n) for i in range(4)])
+    dargs = hip.DeviceBuffer(ctypes.sizeof(args))
+    h = hip._lib()
+    hip.check(h.hipMemcpy(ctypes.c_void_p(dargs.addr), ctypes.addressof(args), ctypes.sizeof(args),
+                          hip.hipMemcpyHostToDevice), "args")
+    s = stream.raw()
+
+    def serial():
+        for i in range(2):
+            hip.check(h.hipMemcpyAsync(ctypes.c_void_p(dsc[i].addr), ctypes.c_void_p(sc[i].addr), 32,
+                                       hip.hipMemcpyHostToDevice, s), "H2D")
+        for i in range(3):
+            hip.check(h.hipMemcpyAsync(ctypes.c_void_p(whole[i].addr), ctypes.c_void_p(host[i].addr), nbytes,
+                                       hip.hipMemcpyHostToDevice, s), "H2D")
+        hip.check(plain.c_func(dargs.addr, dargs.addr + 5 * 40, n, True, None, s), "kernel")
+        hip.check(h.hipMemcpyAsync(ctypes.c_void_p(host[3].addr), ctypes.c_void_p(whole[3].addr), nbytes,
+                                   hip.hipMemcpyDeviceToHost, s), "D2H")
+        stream.sync()
+
+    serial()
+    t = time.perf_counter()
+    for _ in range(steps):
+        serial()
+    serial_ms = (time.perf_counter() - t) * 1e3 / steps
+
+    def copy_rate(kind, dst, src):
+        hip.check(h.hipMemcpyAsync(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, kind, s), "copy")
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            hip.check(h.hipMemcpyAsync(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, kind, s), "copy")
+        stream.sync()
+        return nbytes * steps / (time.perf_counter() - t0) / 1e9
+
+    h2d = copy_rate(hip.hipMemcpyHostToDevice, whole[0].addr, host[0].addr)
+    d2h = copy_rate(hip.hipMemcpyDeviceToHost, host[3].addr, whole[3].addr)
+    moved = 4 * nbytes  # three polynomials in, one out
+    # the link bound: the in-copies (3/4 of the bytes) at the H2D rate overlapping the out-copy at the D2H rate
+    bound_ms = max(3 * nbytes / h2d, nbytes / d2h) / 1e6
+    return {
+        "log_n": log_n,
+        "divide_parts": parts,
+        "pipelined_ms": round(piped_ms, 3),
+        "serial_ms": round(serial_ms, 3),
+        "speedup": round(serial_ms / piped_ms, 3),
+        "pipelined_gbs": round(moved / (piped_ms * 1e-3) / 1e9, 2),
+        "link_h2d_gbs": round(h2d, 2),
+        "link_d2h_gbs": round(d2h, 2),
+        "link_bound_ms": round(bound_ms, 3),
+        "frac_of_link_bound": round(bound_ms / piped_ms, 4),
+    }
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -520,6 +617,11 @@ def main():
     if args.batched_only:
         log("[bench] batched form only")
         print(json.dumps({"batched": batched_leg(libs, stream, args.steps, not args.no_cpu)}), flush=True)
+        return
+    if args.pipeline_only:
+        log("[bench] pipelined fused kernel only")
+        out = {f"parts_{p}": pipeline_leg(libs, stream, args.log_n, p, args.steps) for p in (4, 8, 16, 32)}
+        print(json.dumps({"pipelined_fused": out}), flush=True)
         return
     if args.poly_only:
         log(f"[bench] libpoly.so 2^{args.log_n} only")
@@ -607,6 +709,8 @@ def main():
         line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] libpoly.so at 2^24")
         line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
+        log("[bench] pipelined fused kernel over host memory (2^24, 16 chunks)")
+        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 16, 3)
         log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
         line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
         log("[bench] serving form (Poisson 50..2000 req/s)")

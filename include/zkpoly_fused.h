@@ -11,7 +11,8 @@
  *
  * vars / mut_vars are DEVICE arrays; scalars are 1-element views.  Each
  * library also exports <name>_batched (instance k reads vars[k*NV..],
- * mut_vars[k*NM..] and elements [prefix[k], prefix[k+1])) and <name>_arity.
+ * mut_vars[k*NM..] and elements [prefix[k], prefix[k+1])), <name>_arity and
+ * <name>_mut_reads (one flag per mut_var: 1 when the kernel reads it).
  */
 #ifndef ZKPOLY_FUSED_H
 #define ZKPOLY_FUSED_H
@@ -53,7 +54,10 @@ int zkp_fused_build(const char* const* graph_jsons, const char* const* names, in
  * buffers per read-only polynomial and three per mutable one
  * (dev_polys[j * num_polys + i], dev_mut_polys[j * num_mut_polys + i]), each
  * of len / divide_parts elements; scalars in one-element device buffers,
- * mutable scalars copied back at the end.  Argument arrays are laid out
+ * mutable scalars copied back at the end.  mut_poly_read (NULL: all) marks
+ * the mutable polynomials the kernel reads; the others are not copied in
+ * (<name>_mut_reads of a generated library gives the flags per mut_var).
+ * Argument arrays are laid out
  * [scalars..., polys...] per buffer set in arg_buffer, which holds
  * 2 * (num_scalars + num_polys) + 3 * (num_mut_scalars + num_mut_polys)
  * PolyPtr.  Host polynomials are unsliced (rotation allowed).  Synchronous.
@@ -64,7 +68,8 @@ hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, co
                                unsigned int* const* host_mut_scalars, unsigned int* const* dev_mut_scalars,
                                unsigned int num_polys, const ConstPolyPtr* host_polys, unsigned int* const* dev_polys,
                                unsigned int num_mut_polys, const PolyPtr* host_mut_polys,
-                               unsigned int* const* dev_mut_polys, unsigned long long len, unsigned int divide_parts,
+                               unsigned int* const* dev_mut_polys, const unsigned char* mut_poly_read,
+                               unsigned long long len, unsigned int divide_parts,
                                void* arg_buffer, size_t arg_buffer_size, void* local_buffer, int device);
 
 #ifdef __cplusplus

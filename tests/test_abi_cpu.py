@@ -326,7 +326,8 @@ def test_fused_pipelined_rejects_before_any_work():
     none = (ctypes.c_void_p * 1)()
 
     def call(host_p=host, len_=64, parts=4, argsz=3 * 40, kernel=0x5000, arg=0x6000):
-        return f(kernel, 0, none, none, 0, none, none, 0, None, none, 1, host_p, dev, len_, parts, arg, argsz, None, 0)
+        return f(kernel, 0, none, none, 0, none, none, 0, None, none, 1, host_p, dev, None, len_, parts, arg, argsz,
+                 None, 0)
 
     assert call(parts=3) == HIP_INVALID_VALUE  # divide_parts must exceed 3
     assert call(len_=66) == HIP_INVALID_VALUE  # len a multiple of divide_parts

@@ -33,6 +33,14 @@ def test_codegen_arity_and_symbols():
     assert "out2[0] = 2; out2[1] = 4;" in src
 
 
+def test_codegen_marks_mutable_vars_the_kernel_reads():
+    """Only the in-place output's slot is read; the others are written in full."""
+    src = fused.codegen(Libs(), fused.graph_inplace_mixed(), "k2")
+    assert 'void k2_mut_reads(unsigned char* out) { out[0] = 1; out[1] = 0; out[2] = 0; out[3] = 0;' in src
+    src = fused.codegen(Libs(), fused.graph_ab_gamma_c_minus_z(), "k3")
+    assert 'void k3_mut_reads(unsigned char* out) { out[0] = 0;' in src
+
+
 def test_codegen_coef_rule_text():
     src = fused.codegen(Libs(), fused.graph_coef_rules(), "k3")
     assert src.count("(idx == 0 && is_first)") >= 3 * 2  # single + batched kernels

@@ -332,6 +332,7 @@ struct Emitter {
   std::string err;
   std::map<uint64_t, size_t> var_slot;      // input id -> vars index
   std::map<uint64_t, size_t> mut_slot;      // input/output id -> mut_vars index
+  std::set<size_t> read_slots;              // mut_vars the kernel also reads (in-place inputs)
   std::vector<uint64_t> order;              // topological order of used vertices
   std::set<uint64_t> done;
 
@@ -373,6 +374,7 @@ struct Emitter {
         if (g.v[ov.in_node].mut_input) {
           inplace.insert(ov.in_node);
           mut_slot[ov.in_node] = k;
+          read_slots.insert(k);
         }
       }
     }
@@ -557,6 +559,11 @@ struct Emitter {
     o << "// number of vars / mut_vars the kernel expects\n";
     o << "extern \"C\" void " << name << "_arity(unsigned int* out2) { out2[0] = " << nv << "; out2[1] = " << nm
       << "; }\n";
+    // which mut_vars hold data the kernel reads (in-place inputs); the others
+    // are written in full, so a streaming caller need not load them
+    o << "extern \"C\" void " << name << "_mut_reads(unsigned char* out) {";
+    for (size_t k = 0; k < nm; k++) o << " out[" << k << "] = " << (read_slots.count(k) ? 1 : 0) << ";";
+    o << " (void)out; }\n";
     return o.str();
   }
 };

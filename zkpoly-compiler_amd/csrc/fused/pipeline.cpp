@@ -9,8 +9,8 @@
 // per mutable polynomial (loading, computing, storing back).  Three streams of
 // default priority carry the copies in, the kernels and the copies out;
 // events order them exactly as the reference does:
-//   copy-in of mutable chunk c waits for copy-out of the chunk that last used
-//   its buffer (c - 3); copy-in of read-only chunk c waits for the kernel that
+//   copy-in of mutable chunk c (skipped for polynomials the kernel only
+//   writes) waits for copy-out of the chunk that last used its buffer (c - 3); copy-in of read-only chunk c waits for the kernel that
 //   last used its buffer (c - 2); the kernel waits for both copy-ins; the
 //   copy-out waits for the kernel.
 // Scalars travel once: read-only and mutable scalars in before the first
@@ -96,7 +96,10 @@ extern "C" {
 // Runs `kernel` over host polynomials in divide_parts chunks (see above).
 // Device buffers: dev_polys[j * num_polys + i] (j < 2) and
 // dev_mut_polys[j * num_mut_polys + i] (j < 3) of chunk_len elements each;
-// dev_scalars / dev_mut_scalars of one element each.  arg_buffer holds
+// dev_scalars / dev_mut_scalars of one element each.  mut_poly_read[i] == 0
+// marks a mutable polynomial the kernel only writes (every element of every
+// chunk), whose chunks are then not copied in; NULL copies every one in, as
+// the reference does.  arg_buffer holds
 // (2 * (num_scalars + num_polys) + 3 * (num_mut_scalars + num_mut_polys))
 // PolyPtr; local_buffer is passed to the kernel.  hipErrorInvalidValue, before
 // any work, when a precondition of the reference fails.
@@ -105,7 +108,8 @@ hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, co
                                uint32_t* const* host_mut_scalars, uint32_t* const* dev_mut_scalars,
                                unsigned int num_polys, const ConstPolyPtr* host_polys, uint32_t* const* dev_polys,
                                unsigned int num_mut_polys, const PolyPtr* host_mut_polys,
-                               uint32_t* const* dev_mut_polys, unsigned long long len, unsigned int divide_parts,
+                               uint32_t* const* dev_mut_polys, const unsigned char* mut_poly_read,
+                               unsigned long long len, unsigned int divide_parts,
                                void* arg_buffer, size_t arg_buffer_size, void* local_buffer, int device) {
   const unsigned nv = num_scalars + num_polys, nm = num_mut_scalars + num_mut_polys;
   if (kernel == nullptr || arg_buffer == nullptr || divide_parts <= 3 || len == 0 || len % divide_parts != 0)
@@ -181,6 +185,7 @@ hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, co
     // mutable chunk in, once its buffer's previous copy-out is done
     wait(h2d, mut_d2h[mb]);
     for (unsigned i = 0; i < num_mut_polys; i++) {
+      if (mut_poly_read && !mut_poly_read[i]) continue;  // written in full by the kernel
       const PolyPtr& p = host_mut_polys[i];
       copy_in(err, p.ptr, len, p.rotate, start, chunk, dev_mut_polys[mb * num_mut_polys + i], h2d);
     }

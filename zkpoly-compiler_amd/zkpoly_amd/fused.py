@@ -341,8 +341,8 @@ def _pipeline_lib(libs: Libs) -> ctypes.CDLL:
     lib = libs.load_relative(PIPELINE_LIB)
     if not getattr(lib, "_zkp_typed", False):
         vp, u = ctypes.c_void_p, ctypes.c_uint
-        lib.zkp_fused_pipelined.argtypes = [vp, u, vp, vp, u, vp, vp, u, vp, vp, u, vp, vp, ctypes.c_ulonglong, u,
-                                            vp, ctypes.c_size_t, vp, ctypes.c_int]
+        lib.zkp_fused_pipelined.argtypes = [vp, u, vp, vp, u, vp, vp, u, vp, vp, u, vp, vp, vp, ctypes.c_ulonglong,
+                                            u, vp, ctypes.c_size_t, vp, ctypes.c_int]
         lib.zkp_fused_pipelined.restype = ctypes.c_int
         lib._zkp_typed = True
     return lib
@@ -368,12 +368,21 @@ class PipelinedFusedKernel(RegisteredFunction):
         self.kernel = FusedKernel(libs, meta)
         self.meta = meta
         self.driver = _pipeline_lib(libs).zkp_fused_pipelined
+        # mut_vars the kernel reads; write-only outputs are not streamed in
+        reads = (ctypes.c_ubyte * max(1, meta.num_mut_vars))(*([1] * meta.num_mut_vars))
+        f = getattr(self.kernel.lib, meta.name + "_mut_reads", None)
+        if f is not None:
+            f.argtypes = [ctypes.c_void_p]
+            f.restype = None
+            f(reads)
+        self.mut_reads = [int(x) for x in reads[:meta.num_mut_vars]]
 
     def get_fn(self) -> Function:
         meta, driver = self.meta, self.driver
         pm = meta.pipelined_meta
         kernel_addr = ctypes.cast(self.kernel.c_func, ctypes.c_void_p).value
         p, q, parts = pm.num_mut_scalars, pm.num_scalars, pm.divide_parts
+        poly_reads = self.mut_reads[p:]  # mut_vars are [mut scalars..., mut polys...]
 
         def f(mut_var, var, gpu_mapping=None):
             arg_buffer, local_buffer = mut_var[0], mut_var[1]
@@ -414,8 +423,9 @@ class PipelinedFusedKernel(RegisteredFunction):
             dp = arr(c_vp, [b.ptr for b in poly_bufs])
             hmp = arr(PolyPtr, [PolyPtr.from_array(a) for a in mut_polys])
             dmp = arr(c_vp, [b.ptr for b in mut_poly_bufs])
+            reads = arr(ctypes.c_ubyte, poly_reads)
             device = gpu_mapping(0) if gpu_mapping is not None else 0
-            hip.check(driver(kernel_addr, q, hs, ds, p, hms, dms, m, hp, dp, n, hmp, dmp, length, parts,
+            hip.check(driver(kernel_addr, q, hs, ds, p, hms, dms, m, hp, dp, n, hmp, dmp, reads, length, parts,
                              arg_buffer.ptr, arg_buffer.size, local_buffer.ptr, device), meta.name + " (pipelined)")
 
         return Function(FuncMeta(meta.name, KernelType.FusedArith, meta), f)
