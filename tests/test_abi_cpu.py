@@ -230,14 +230,15 @@ def test_poly_buffer_size_queries(n):
     """The NULL-buffer size protocol of core/src/poly.rs:265-280, 364-378,
     413-427, 570-586, 631-647: the size comes back and nothing is launched.
     Scans keep one 32-byte product (64-byte Horner pair for eval / kate) per
-    2048-element tile, twice; batched_invert adds the n prefix products."""
+    2048-element tile, twice; batched_invert keeps the tile products and the
+    products before and after each tile, and the inverse of the total."""
     from zkpoly_amd.poly import KateDivision, PolyEval, PolyInvert, PolyScan
 
     libs = Libs()
     tiles = -(-n // 2048)
     assert PolyScan(libs).get_buffer_size(n) == 2 * _a256(tiles * 32)
     assert PolyEval(libs).get_buffer_size(n) == _a256(tiles * 64) + 256
-    assert PolyInvert(libs).get_buffer_size(n) == _a256(n * 32) + 2 * _a256(tiles * 32) + 2 * 256
+    assert PolyInvert(libs).get_buffer_size(n) == 3 * _a256(tiles * 32) + 256
     if n & (n - 1) == 0:
         k = n.bit_length() - 1
         assert KateDivision(libs).get_buffer_size(k) == 2 * _a256(tiles * 64)

@@ -11,7 +11,8 @@
 //    (poly_basic.cu:65-73);
 //  * scan_mul: in-place inclusive prefix product (scan_mul.cuh:7-27);
 //  * batched_invert: x_i^-1 for x_i != 0, zero stays zero
-//    (batched_invert.cuh:49-91);
+//    (batched_invert.cuh:49-91), by Montgomery's trick over tiles (a tile
+//    fold, one workgroup for the carries and the one inversion, a rescan);
 //  * poly_eval: sum_i a_i x^i into a device element (poly_eval.cuh:8-71);
 //  * kate_division: q_i = sum_{j>i} p_j b^(j-i-1), i.e. the quotient of
 //    (p(X) - p(b)) / (X - b), with q_{n-1} = 0 (kate_division.cuh:55-83).
@@ -125,30 +126,6 @@ struct SrcNonZero {  // x_k, or 1 for x_k = 0
     return ProdOp::T{x.is_zero() ? Fr::one() : x};
   }
 };
-struct SrcNonZeroRev {  // x_{n-1-k}, or 1 for zero
-  PolyView v;
-  __device__ ProdOp::T operator()(uint64_t k) const {
-    const Fr x = ld(v, v.len - 1 - k);
-    return ProdOp::T{x.is_zero() ? Fr::one() : x};
-  }
-};
-struct SinkStoreExcl {  // pre[k] = product of the non-zero x_j, j < k
-  Fr* pre;
-  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&, const ProdOp::T&) const {
-    pre[k] = excl.v;
-  }
-};
-struct SinkInvert {  // x_i = pre[i] * suf[i] / total for x_i != 0 (i = n-1-k)
-  PolyView v;
-  const Fr* pre;
-  const Fr* inv_total;
-  __device__ void operator()(uint64_t k, const ProdOp::T& excl, const ProdOp::T&, const ProdOp::T&) const {
-    const uint64_t i = v.len - 1 - k;
-    const Fr x = ld(v, i);
-    st(v, i, x.is_zero() ? Fr::zero() : pre[i] * excl.v * (*inv_total));
-  }
-};
-
 struct SrcHorner {  // (a_k, x)
   PolyView v;
   const uint32_t* x;
@@ -166,8 +143,78 @@ struct SinkKate {  // q_{n-1-k} = S of the fold of (p_j, b), j > n-1-k
   }
 };
 
-__global__ void k_invert_total(const Fr* total, Fr* inv) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *inv = total->inv_binary();
+// ---------------------------------------------------------------- batched inversion
+//
+// Montgomery's trick over tiles, zeros read as one: after the tile fold of the
+// non-zero elements, one workgroup forms for every tile the products of the
+// tiles before (pre) and after (suf) it and the inverse of the whole product;
+// then every thread of a tile rescans its 8 elements, so that
+//   x_i^-1 = (product of non-zero x_j, j < i) * (product of non-zero x_j, j > i) * total^-1.
+
+// One workgroup: pre[b], suf[b] and *inv = (product of all tile folds)^-1.
+__global__ void __launch_bounds__(TH) k_invert_carries(const ProdOp::T* __restrict__ partial, uint64_t nb,
+                                                       Fr* __restrict__ pre, Fr* __restrict__ suf, Fr* inv) {
+  __shared__ ProdOp::T sh[TH];
+  const uint64_t per = (nb + TH - 1) / TH;
+  const uint64_t lo = (uint64_t)threadIdx.x * per;
+  const uint64_t hi = lo + per < nb ? lo + per : nb;
+  Fr v = Fr::one();
+  for (uint64_t j = lo; j < hi; j++) v = v * partial[j].v;
+  ProdOp::T tot;
+  Fr acc = block_exclusive<ProdOp>(ProdOp::T{v}, sh, &tot).v;
+  for (uint64_t j = lo; j < hi; j++) {
+    pre[j] = acc;
+    acc = acc * partial[j].v;
+  }
+  // products commute: scanning the threads in mirrored order gives each the
+  // product of the later segments
+  acc = block_exclusive<ProdOp>(ProdOp::T{v}, sh, &tot, TH - 1 - threadIdx.x).v;
+  for (uint64_t j = hi; j-- > lo;) {
+    suf[j] = acc;
+    acc = acc * partial[j].v;
+  }
+  if (threadIdx.x == 0) *inv = tot.v.inv_binary();
+}
+
+// Tile rescan: every thread holds its 8 elements (zeros as one), takes the
+// products of the threads before and after it within the tile, and forms each
+// inverse from its left and right products: 4 products per element.
+__global__ void __launch_bounds__(TH) k_invert_tile(PolyView v, uint64_t n, const Fr* __restrict__ pre,
+                                                    const Fr* __restrict__ suf, const Fr* __restrict__ inv) {
+  __shared__ ProdOp::T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  Fr x[PER];
+  Fr t = Fr::one();
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    x[j] = k < n ? ld(v, k) : Fr::one();
+    if (x[j].is_zero()) x[j] = Fr::one();
+    t = j ? t * x[j] : x[j];
+  }
+  ProdOp::T tot;
+  const Fr before = block_exclusive<ProdOp>(ProdOp::T{t}, sh, &tot).v;
+  const Fr after = block_exclusive<ProdOp>(ProdOp::T{t}, sh, &tot, TH - 1 - threadIdx.x).v;
+  Fr left, right;
+  Fr::mul2(pre[blockIdx.x], before, after, suf[blockIdx.x], left, right);
+  right = right * (*inv);
+  // x[j] <- product of everything after element j
+#pragma unroll
+  for (int j = PER - 1; j >= 0; j--) {
+    const Fr e = x[j];
+    x[j] = right;
+    if (j) right = right * e;
+  }
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k < n) {
+      const Fr y = ld(v, k);  // read again: x[j] now holds the right product
+      const bool zero = y.is_zero();
+      st(v, k, zero ? Fr::zero() : left * x[j]);
+      if (j + 1 < PER && !zero) left = left * y;
+    }
+  }
 }
 
 __global__ void k_store_s(const HornerOp::T* total, uint32_t* res) {
@@ -306,29 +353,25 @@ hipError_t batched_invert(void* temp_buffer, unsigned long* buffer_size, PolyPtr
   if (!make_view(poly, &v)) return hipErrorInvalidValue;
   const uint64_t n = v.len;
   const uint64_t nb = tiles_of(n ? n : 1);
-  const size_t pre = align256((n ? n : 1) * sizeof(Fr));
-  const size_t part = align256(nb * sizeof(ProdOp::T));
-  const size_t one = align256(sizeof(Fr));
+  const size_t part = align256(nb * sizeof(Fr));
   if (temp_buffer == nullptr) {
     if (buffer_size == nullptr) return hipErrorInvalidValue;
-    *buffer_size = pre + 2 * part + 2 * one;
+    *buffer_size = 3 * part + align256(sizeof(Fr));
     return hipSuccess;
   }
   if (n == 0) return hipSuccess;
   if (v.ptr == nullptr) return hipErrorInvalidValue;
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
   char* t = reinterpret_cast<char*>(temp_buffer);
-  Fr* prefix = reinterpret_cast<Fr*>(t);
-  auto* partial = reinterpret_cast<ProdOp::T*>(t + pre);
-  auto* carry = reinterpret_cast<ProdOp::T*>(t + pre + part);
-  auto* total = reinterpret_cast<ProdOp::T*>(t + pre + 2 * part);
-  Fr* inv = reinterpret_cast<Fr*>(t + pre + 2 * part + one);
-  // prefix[i] = product of the non-zero x_j, j < i; total = product of all of them
-  hipError_t err = run_scan<ProdOp>(SrcNonZero{v}, SinkStoreExcl{prefix}, n, partial, carry, total, stream);
-  if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(k_invert_total, dim3(1), dim3(1), 0, stream, &total->v, inv);
-  // the suffix products (mirrored scan) finish x_i^-1 = prefix_i * suffix_i / total
-  return run_scan<ProdOp>(SrcNonZeroRev{v}, SinkInvert{v, prefix, inv}, n, partial, carry, (ProdOp::T*)nullptr,
-                          stream);
+  auto* partial = reinterpret_cast<ProdOp::T*>(t);
+  Fr* pre = reinterpret_cast<Fr*>(t + part);
+  Fr* suf = reinterpret_cast<Fr*>(t + 2 * part);
+  Fr* inv = reinterpret_cast<Fr*>(t + 3 * part);
+  hipLaunchKernelGGL((k_tile_fold<ProdOp, SrcNonZero>), dim3((unsigned)nb), dim3(TH), 0, stream, SrcNonZero{v}, n,
+                     partial);
+  hipLaunchKernelGGL(k_invert_carries, dim3(1), dim3(TH), 0, stream, partial, nb, pre, suf, inv);
+  hipLaunchKernelGGL(k_invert_tile, dim3((unsigned)nb), dim3(TH), 0, stream, v, n, pre, suf, inv);
+  return hipGetLastError();
 }
 
 // *target = *target^-1, zero stays zero (poly.h:24)

@@ -38,25 +38,32 @@ __device__ __forceinline__ typename Op::T thread_fold(const Src& src, uint64_t k
   return acc;
 }
 
-// Exclusive scan of one value per thread in thread order (Hillis-Steele in
-// LDS); *total = fold of all of them.  Every thread of the block calls it.
+// Exclusive scan of one value per thread in the order of `pos` (a permutation
+// of the thread indices; thread order by default), Hillis-Steele in LDS;
+// *total = fold of all of them.  Every thread of the block calls it.
 template <class Op>
 __device__ __forceinline__ typename Op::T block_exclusive(const typename Op::T& v, typename Op::T* sh,
-                                                          typename Op::T* total) {
+                                                          typename Op::T* total, uint32_t pos) {
   using T = typename Op::T;
-  sh[threadIdx.x] = v;
+  sh[pos] = v;
   __syncthreads();
   for (uint32_t off = 1; off < TH; off <<= 1) {
-    T cur = sh[threadIdx.x];
-    if (threadIdx.x >= off) cur = Op::comb(sh[threadIdx.x - off], cur);
+    T cur = sh[pos];
+    if (pos >= off) cur = Op::comb(sh[pos - off], cur);
     __syncthreads();
-    sh[threadIdx.x] = cur;
+    sh[pos] = cur;
     __syncthreads();
   }
-  const T excl = threadIdx.x ? sh[threadIdx.x - 1] : Op::id();
+  const T excl = pos ? sh[pos - 1] : Op::id();
   *total = sh[TH - 1];
   __syncthreads();  // sh is reused by the caller
   return excl;
+}
+
+template <class Op>
+__device__ __forceinline__ typename Op::T block_exclusive(const typename Op::T& v, typename Op::T* sh,
+                                                          typename Op::T* total) {
+  return block_exclusive<Op>(v, sh, total, threadIdx.x);
 }
 
 // partial[b] = fold of tile b.
