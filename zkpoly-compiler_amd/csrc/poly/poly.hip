@@ -126,15 +126,52 @@ struct SrcNonZero {  // x_k, or 1 for x_k = 0
     return ProdOp::T{x.is_zero() ? Fr::one() : x};
   }
 };
+// (sum_t v_{i0+t} x^t, x^cnt) over t < cnt <= PER: Horner from the top (cnt - 1
+// products) and x^cnt by squaring, instead of two products per element.
+__device__ __forceinline__ HornerOp::T horner_run(const PolyView& v, uint64_t i0, uint32_t cnt, const Fr& x) {
+  if (cnt == 0) return HornerOp::id();
+  Fr e[PER];
+#pragma unroll
+  for (int t = 0; t < PER; t++)
+    if ((uint32_t)t < cnt) e[t] = ld(v, i0 + t);
+  Fr s = e[0];
+#pragma unroll
+  for (int t = PER - 1; t >= 0; t--) {
+    if ((uint32_t)t + 1 == cnt) s = e[t];
+    else if ((uint32_t)t + 1 < cnt) s = s * x + e[t];
+  }
+  Fr m = x, base = x;  // x^cnt, cnt >= 1
+  bool started = false;
+  for (uint32_t c = cnt; c; c >>= 1) {
+    if (c & 1) {
+      m = started ? m * base : base;
+      started = true;
+    }
+    if (c >> 1) base = base.sqr();
+  }
+  return HornerOp::T{s, m};
+}
+
 struct SrcHorner {  // (a_k, x)
   PolyView v;
   const uint32_t* x;
   __device__ HornerOp::T operator()(uint64_t k) const { return HornerOp::T{ld(v, k), Fr::load(x)}; }
+  // elements k0 .. k0+cnt-1 in order: a contiguous run of a
+  __device__ HornerOp::T fold(uint64_t k0, uint64_t n) const {
+    const uint32_t cnt = k0 < n ? (uint32_t)(n - k0 < (uint64_t)PER ? n - k0 : PER) : 0;
+    return horner_run(v, k0, cnt, Fr::load(x));
+  }
 };
 struct SrcHornerRev {  // (p_{n-1-k}, b)
   PolyView v;
   const uint32_t* b;
   __device__ HornerOp::T operator()(uint64_t k) const { return HornerOp::T{ld(v, v.len - 1 - k), Fr::load(b)}; }
+  // under RevOp the run k0 .. k0+cnt-1 folds as the Horner run of
+  // p_{n-k0-cnt} .. p_{n-1-k0} in ascending index order
+  __device__ HornerOp::T fold(uint64_t k0, uint64_t n) const {
+    const uint32_t cnt = k0 < n ? (uint32_t)(n - k0 < (uint64_t)PER ? n - k0 : PER) : 0;
+    return horner_run(v, n - k0 - cnt, cnt, Fr::load(b));
+  }
 };
 struct SinkKate {  // q_{n-1-k} = S of the fold of (p_j, b), j > n-1-k
   PolyView q;

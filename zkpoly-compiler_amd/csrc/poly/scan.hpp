@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../common/poly_ptr.hpp"
 #include "../field/bn254.hpp"
 
@@ -26,16 +28,27 @@ __device__ __forceinline__ void st(const PolyView& v, uint64_t i, const Fr& x) {
 
 // ---------------------------------------------------------------- scan machinery
 
+// A source may fold a thread's run itself (src.fold(k0, n)), when it knows a
+// cheaper way than element-by-element combines.
+template <class Src, class = void>
+struct has_fold : std::false_type {};
+template <class Src>
+struct has_fold<Src, std::void_t<decltype(&Src::fold)>> : std::true_type {};
+
 // Fold of elements [k0, min(k0 + PER, n)) in order.
 template <class Op, class Src>
 __device__ __forceinline__ typename Op::T thread_fold(const Src& src, uint64_t k0, uint64_t n) {
-  typename Op::T acc = Op::id();
-  for (int j = 0; j < PER; j++) {
-    const uint64_t k = k0 + j;
-    if (k >= n) break;
-    acc = Op::comb(acc, src(k));
+  if constexpr (has_fold<Src>::value) {
+    return src.fold(k0, n);
+  } else {
+    typename Op::T acc = Op::id();
+    for (int j = 0; j < PER; j++) {
+      const uint64_t k = k0 + j;
+      if (k >= n) break;
+      acc = Op::comb(acc, src(k));
+    }
+    return acc;
   }
-  return acc;
 }
 
 // Exclusive scan of one value per thread in the order of `pos` (a permutation
