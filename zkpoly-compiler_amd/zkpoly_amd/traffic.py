@@ -150,7 +150,7 @@ def summarize_trace(trace_dir: str, title: str, command: str, out_md: str, tail_
              "## Kernel stats (all dispatches of the run)", "",
              "| kernel | calls | avg ms | min ms | max ms | % time |", "|---|---|---|---|---|---|"]
     for r in stats:
-        name = r["Name"].split("(")[0]
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0]
         lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | {float(r['MinNs']) / 1e6:.4f} | "
                      f"{float(r['MaxNs']) / 1e6:.4f} | {float(r['Percentage']):.1f} |")
     if tail_kernel and tail_n:
