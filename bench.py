@@ -709,8 +709,8 @@ def main():
         line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] libpoly.so at 2^24")
         line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
-        log("[bench] pipelined fused kernel over host memory (2^24, 16 chunks)")
-        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 16, 3)
+        log("[bench] pipelined fused kernel over host memory (2^24, 8 chunks)")
+        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 8, 3)
         log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
         line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
         log("[bench] serving form (Poisson 50..2000 req/s)")
