@@ -471,10 +471,20 @@ def poly_leg(libs, stream, log_n, steps, with_cpu):
         cpu_s = time.perf_counter() - t
         out["cpu_baseline_kate_division"] = {"value": round(m / cpu_s / 1e9, 5), "unit": "Gelem/s", "cores": 1,
                                              "kind": "port", "sample": f"kate_division of 2^{m.bit_length() - 1}"}
+        mp = 1 << 20
+        pool = random_elements(mp // 2, 93)
+        rng = np.random.default_rng(94)
+        tab = pool[rng.integers(0, mp // 2, mp)]
+        inp = tab[rng.integers(0, mp, mp)]
+        t = time.perf_counter()
+        orc.permute(inp, tab, mp)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline_permute"] = {"value": round(mp / cpu_s / 1e9, 5), "unit": "Gelem/s", "cores": 1,
+                                       "kind": "port", "sample": "permute of 2^20 rows (same value distribution)"}
     return out
 
 
-def pipeline_leg(libs, stream, log_n, parts, steps):
+def pipeline_leg(libs, stream, log_n, parts, steps, with_cpu=False):
     """PipelinedFusedKernel: out = a*b + gamma*c - z over 2^log_n host-resident
     (pinned) elements in `parts` chunks, against one whole copy-in, kernel and
     copy-out on a single stream, and the plain pinned copy rates of the link."""
@@ -556,7 +566,23 @@ def pipeline_leg(libs, stream, log_n, parts, steps):
     moved = 4 * nbytes  # three polynomials in, one out
     # the link bound: the in-copies (3/4 of the bytes) at the H2D rate overlapping the out-copy at the D2H rate
     bound_ms = max(3 * nbytes / h2d, nbytes / d2h) / 1e6
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        m = 1 << 20  # the same expression on the host, one core, over a 2^20 sample of the inputs
+        a, b, c = (host[i].view()[: 4 * m].reshape(m, 4) for i in range(3))
+        g = np.broadcast_to(sc[0].view(), (m, 4)).copy()
+        z = np.broadcast_to(sc[1].view(), (m, 4)).copy()
+        t = time.perf_counter()
+        orc.vec("orc_vec_sub", orc.FR, orc.vec("orc_vec_add", orc.FR, orc.vec("orc_vec_mul", orc.FR, a, b),
+                                                orc.vec("orc_vec_mul", orc.FR, g, c)), z)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(m / cpu_s / 1e9, 5), "unit": "Gelem/s", "cores": 1, "kind": "port",
+               "sample": "a*b + gamma*c - z over 2^20 host elements"}
     return {
+        "cpu_baseline": cpu,
         "log_n": log_n,
         "divide_parts": parts,
         "pipelined_ms": round(piped_ms, 3),
@@ -710,7 +736,7 @@ def main():
         log("[bench] libpoly.so at 2^24")
         line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] pipelined fused kernel over host memory (2^24, 8 chunks)")
-        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 8, 3)
+        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 8, 3, not args.no_cpu)
         log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
         line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
         log("[bench] serving form (Poisson 50..2000 req/s)")
