@@ -484,22 +484,33 @@ def poly_leg(libs, stream, log_n, steps, with_cpu):
     return out
 
 
-def pipeline_leg(libs, stream, log_n, parts, steps, with_cpu=False):
-    """PipelinedFusedKernel: out = a*b + gamma*c - z over 2^log_n host-resident
-    (pinned) elements in `parts` chunks, against one whole copy-in, kernel and
-    copy-out on a single stream, and the plain pinned copy rates of the link."""
-    from zkpoly_amd import fused, hip
+def pipeline_host(log_n):
+    """The pinned host arrays of the pipelined leg (a, b, c, out; gamma, z).
+    A runtime pins its host pool once at start-up (the reference's CPU memory
+    pool does), so main() allocates these before the other legs run."""
+    from zkpoly_amd import hip
     from zkpoly_amd.field import random_elements
-    from zkpoly_amd.runtime import GpuBuffer, PolyPtr, Scalar, ScalarArray
 
     n = 1 << log_n
-    nbytes = n * 32
-    host = [hip.PinnedBuffer(nbytes) for _ in range(4)]  # a, b, c, out
+    host = [hip.PinnedBuffer(n * 32) for _ in range(4)]
     for i in range(3):
         host[i].view()[:] = random_elements(n, 500 + i).reshape(-1)
     sc = [hip.PinnedBuffer(32) for _ in range(2)]
     for i in range(2):
         sc[i].view()[:] = random_elements(1, 510 + i)[0]
+    return host, sc
+
+
+def pipeline_leg(libs, stream, log_n, parts, steps, with_cpu=False, pinned=None):
+    """PipelinedFusedKernel: out = a*b + gamma*c - z over 2^log_n host-resident
+    (pinned) elements in `parts` chunks, against one whole copy-in, kernel and
+    copy-out on a single stream, and the plain pinned copy rates of the link."""
+    from zkpoly_amd import fused, hip
+    from zkpoly_amd.runtime import GpuBuffer, PolyPtr, Scalar, ScalarArray
+
+    n = 1 << log_n
+    nbytes = n * 32
+    host, sc = pinned if pinned is not None else pipeline_host(log_n)
     meta = fused.FusedKernelMeta("fused_arith_pipe_abgcz", 5, 1, fused.DEMO_DIR, fused.PipelinedMeta(parts, 2, 0))
     k = fused.PipelinedFusedKernel(libs, meta)
     chunk = nbytes // parts
@@ -646,7 +657,9 @@ def main():
         return
     if args.pipeline_only:
         log("[bench] pipelined fused kernel only")
-        out = {f"parts_{p}": pipeline_leg(libs, stream, args.log_n, p, args.steps) for p in (4, 8, 16, 32)}
+        pinned = pipeline_host(args.log_n)
+        out = {f"parts_{p}": pipeline_leg(libs, stream, args.log_n, p, args.steps, pinned=pinned)
+               for p in (4, 8, 16, 32)}
         print(json.dumps({"pipelined_fused": out}), flush=True)
         return
     if args.poly_only:
@@ -659,6 +672,8 @@ def main():
         return
     k = args.log_n
     n = 1 << k
+    # the pipelined leg's host pool, pinned before anything else runs
+    pinned = None if args.quick else pipeline_host(24)
     log(f"[bench] headline: forward NTT 2^{k}, warmup {args.warmup}, steps {args.steps}")
     w, gpu_out, ms_step, per_kind, nkern = headline(libs, stream, args)
     dit = per_kind[1]
@@ -709,6 +724,28 @@ def main():
         "valu_rocprof": valu_from_profiles(["dit_pass_r4", "bitrev_head_stream", "k_accumulate0", "k_close_runs",
                                             "k_reduce_level", "k_reduce_tail", "k_seg_scatter"]),
     }
+    if not args.quick:
+        # GPU legs first: in this process, after the 16-thread CPU NTT baseline of
+        # 2^24 the pipelined leg's copies stopped overlapping (39 vs 30 ms), so the
+        # multi-threaded CPU baselines come after it
+        host_in = w.host
+        del w
+        log("[bench] config-2 suite")
+        line["suite"] = suite(libs, stream, max(5, args.steps // 2))
+        log("[bench] libpoly.so at 2^24")
+        line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
+        log("[bench] pipelined fused kernel over host memory (2^24, 8 chunks)")
+        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 8, 3, not args.no_cpu, pinned)
+        del pinned
+        log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
+        line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
+        log("[bench] serving form (Poisson 50..2000 req/s)")
+        line["serving"] = serving_leg(libs, stream, (50, 100, 200, 500, 1000, 2000), args.serve_count,
+                                      not args.no_cpu)
+        log("[bench] MSM 2^20 window sweep")
+        line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
+    else:
+        host_in = w.host
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -716,7 +753,7 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline: oracle best_fft 2^{k} on {threads} threads")
         t = time.perf_counter()
-        want = orc.ntt(w.host, k, threads)
+        want = orc.ntt(host_in, k, threads)
         cpu_s = time.perf_counter() - t
         line["cpu_baseline"] = {
             "value": round(n / cpu_s / 1e9, 5),
@@ -727,21 +764,6 @@ def main():
             "seconds": round(cpu_s, 3),
         }
         line["bit_exact"] = bool(np.array_equal(gpu_out, want))
-    if not args.quick:
-        log("[bench] config-2 suite")
-        del w
-        line["suite"] = suite(libs, stream, max(5, args.steps // 2))
-        log("[bench] MSM 2^20 window sweep")
-        line["msm"] = msm_leg(libs, stream, args.msm_log_n, max(3, args.steps // 4), not args.no_cpu)
-        log("[bench] libpoly.so at 2^24")
-        line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
-        log("[bench] pipelined fused kernel over host memory (2^24, 8 chunks)")
-        line["pipelined_fused"] = pipeline_leg(libs, stream, 24, 8, 3, not args.no_cpu)
-        log("[bench] batched form (64 NTT + fused, one hipGraph launch)")
-        line["batched"] = batched_leg(libs, stream, max(5, args.steps // 2), not args.no_cpu)
-        log("[bench] serving form (Poisson 50..2000 req/s)")
-        line["serving"] = serving_leg(libs, stream, (50, 100, 200, 500, 1000, 2000), args.serve_count,
-                                      not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -60,7 +60,9 @@ int zkp_fused_build(const char* const* graph_jsons, const char* const* names, in
  * Argument arrays are laid out
  * [scalars..., polys...] per buffer set in arg_buffer, which holds
  * 2 * (num_scalars + num_polys) + 3 * (num_mut_scalars + num_mut_polys)
- * PolyPtr.  Host polynomials are unsliced (rotation allowed).  Synchronous.
+ * PolyPtr.  Host polynomials are unsliced (rotation allowed).  When every
+ * host polynomial is mapped pinned memory (hipHostMalloc) the kernel runs once
+ * over the host arrays directly and the chunk buffers go unused.  Synchronous.
  * Replaces PipelinedFusedKernel::get_fn, core/src/fused_kernels.rs:198-516
  * (host code in the reference's runtime). */
 hipError_t zkp_fused_pipelined(zkp_fused_fn kernel, unsigned int num_scalars, const unsigned int* const* host_scalars,

@@ -341,3 +341,62 @@ def test_pipelined_coefficient_rules_see_element_zero_once(stream):
     w3 = orc.vec("orc_vec_mul", FR, sv, p)
     for i, w in enumerate((w0, w1, w2, w3)):
         assert np.array_equal(outs[i].logical(), w), i
+
+
+class PageablePoly:
+    """A host polynomial in ordinary (pageable) numpy memory: the driver stages
+    every chunk through device buffers, write-only outputs included."""
+
+    def __init__(self, data, rotate=0):
+        n = data.shape[0]
+        self.buf = np.zeros((n, 4), dtype=np.uint64)
+        self.arr = ScalarArray(n, self.buf.ctypes.data, device="cpu")
+        self.arr.rotate(rotate)
+        self.idx = (np.arange(n) + n - self.arr.get_rotation()) % n
+        self.buf[self.idx] = data
+
+    def logical(self):
+        return self.buf[self.idx].copy()
+
+
+class PageableScalar:
+    """A host scalar in pageable numpy memory."""
+
+    def __init__(self, value):
+        self.value = np.array(value, dtype=np.uint64)
+        self.s = Scalar(self.value.ctypes.data, device="cpu")
+
+
+@pytest.mark.gpu
+def test_pipelined_pageable_host_memory(stream):
+    n, parts = 8 * 2048, 8
+    a, b, c = (random_elements(n, 340 + i) for i in range(3))
+    gamma, z = random_elements(1, 350)[0], random_elements(1, 351)[0]
+    out = PageablePoly(np.zeros((n, 4), np.uint64), 333)
+    k = pipelined("fused_arith_pipe_abgcz", 5, 1, parts, 2, 0)
+    run_pipelined(k, parts, [PageableScalar(gamma), PageableScalar(z)],
+                  [PageablePoly(a, 5), PageablePoly(b), PageablePoly(c, 77)], [], [out], n)
+    g = np.broadcast_to(gamma, (n, 4)).copy()
+    zz = np.broadcast_to(z, (n, 4)).copy()
+    want = orc.vec("orc_vec_sub", FR, orc.vec("orc_vec_add", FR, orc.vec("orc_vec_mul", FR, a, b),
+                                              orc.vec("orc_vec_mul", FR, g, c)), zz)
+    assert np.array_equal(out.logical(), want)
+
+
+@pytest.mark.gpu
+def test_pipelined_mixed_memory(stream):
+    """Staged chunks (a pageable in-place polynomial) next to a pinned write-only
+    output, which the kernel writes directly in each chunk."""
+    n, parts = 6 * 4096, 6
+    x, y = random_elements(n, 360), random_elements(n, 361)
+    s = random_elements(1, 362)[0]
+    hx, hy = PageablePoly(x, 4095), HostPoly(y, 12)
+    hq = HostPoly(np.zeros((n, 4), np.uint64), n - 1)
+    ht = HostScalar(np.zeros(4, np.uint64))
+    k = pipelined("fused_arith_pipe_mixed", 2, 3, parts, 1, 1)
+    run_pipelined(k, parts, [PageableScalar(s)], [hy], [ht], [hx, hq], n)
+    t = orc.add(FR, orc.mul(FR, s, s), s)
+    assert np.array_equal(ht.value(), t)
+    tv = np.broadcast_to(t, (n, 4)).copy()
+    assert np.array_equal(hx.logical(), orc.vec("orc_vec_mul", FR, tv, orc.vec("orc_vec_add", FR, x, y)))
+    assert np.array_equal(hq.logical(), orc.vec("orc_vec_neg", FR, orc.vec("orc_vec_sub", FR, x, y)))
