@@ -339,3 +339,18 @@ def test_fused_pipelined_rejects_before_any_work():
     assert call(host_p=sliced) == HIP_INVALID_VALUE  # "pipelined fused kernel doesn't support slice"
     short = (PolyPtr * 1)(PolyPtr(0x1000, 32, 0, 0, 32))
     assert call(host_p=short) == HIP_INVALID_VALUE
+
+
+def test_poly_operators_refuse_small_temporary_buffers():
+    """A temporary buffer below the size query is refused on the host, before
+    any launch (the library itself trusts the size, as the reference's does)."""
+    from zkpoly_amd.poly import PolyEval, PolyInvert
+    from zkpoly_amd.runtime import GpuBuffer, Scalar
+
+    arr = ScalarArray(4096, 0x1000)
+    inv = PolyInvert(Libs())
+    with pytest.raises(AssertionError, match="too small"):
+        inv.get_fn().f([GpuBuffer(0x2000, inv.get_buffer_size(4096) - 1), arr], [None])
+    ev = PolyEval(Libs())
+    with pytest.raises(AssertionError, match="too small"):
+        ev.get_fn().f([GpuBuffer(0x2000, 16), Scalar(0x3000)], [arr, Scalar(0x4000), None])

@@ -15,7 +15,8 @@ Same operator names, argument lists and checks as the reference's wrappers
   PolyPermute:  mut_var=[res_input, res_table, temp_buf], var=[input, table, stream];
                 usable = |res_input| = |res_table|; get_buffer_size(usable)
 Temporary buffers are runtime.GpuBuffer objects sized by get_buffer_size
-(the library's own size query: a NULL buffer returns the size, runs nothing).
+(the library's own size query: a NULL buffer returns the size, runs nothing);
+a smaller buffer is refused before anything is launched.
 
 The reference's closures also accept host-resident arrays and scalars (var
 empty) and compute on the CPU; this package computes on the GPU only, so such
@@ -197,6 +198,7 @@ class PolyEval(_PolyFunc):
             temp = unwrap(mut_var[0], GpuBuffer, "GpuBuffer")
             res = _gpu_scalar(mut_var[1], "poly_eval")
             poly = _gpu_array(var[0], "poly_eval")
+            assert temp.size >= self.get_buffer_size(poly.len), "temporary buffer too small"
             x = _gpu_scalar(var[1], "poly_eval")
             stream = unwrap_stream(var[2])
             hip.set_device(stream.get_device())
@@ -225,6 +227,7 @@ class KateDivision(_PolyFunc):
             assert res.len == p.len
             assert p.len > 0 and (p.len & (p.len - 1)) == 0, "len must be a power of two"
             log_p = p.len.bit_length() - 1
+            assert temp.size >= self.get_buffer_size(log_p), "temporary buffer too small"
             b = _gpu_scalar(var[1], "kate_division")
             stream = unwrap_stream(var[2])
             hip.set_device(stream.get_device())
@@ -256,6 +259,7 @@ class PolyScan(_PolyFunc):
             target = _gpu_array(mut_var[1], "scan_mul")
             p = _gpu_array(var[0], "scan_mul")
             assert target.len == p.len
+            assert temp.size >= self.get_buffer_size(target.len), "temporary buffer too small"
             x0 = _gpu_scalar(var[1], "scan_mul")
             stream = unwrap_stream(var[2])
             hip.set_device(stream.get_device())
@@ -287,6 +291,7 @@ class PolyInvert(_PolyFunc):
             assert len(var) == 1
             temp = unwrap(mut_var[0], GpuBuffer, "GpuBuffer")
             target = _gpu_array(mut_var[1], "batched_invert")
+            assert temp.size >= self.get_buffer_size(target.len), "temporary buffer too small"
             stream = unwrap_stream(var[0])
             hip.set_device(stream.get_device())
             hip.check(c_func(temp.ptr, None, PolyPtr.from_array(target), stream.raw()), "batched_invert")
@@ -363,6 +368,7 @@ class PolyPermute(_PolyFunc):
             temp = unwrap(mut_var[2], GpuBuffer, "GpuBuffer")
             inp = _gpu_array(var[0], "permute")
             table = _gpu_array(var[1], "permute")
+            assert temp.size >= self.get_buffer_size(res_input.len), "temporary buffer too small"
             stream = unwrap_stream(var[2])
             hip.set_device(stream.get_device())
             hip.check(c_func(temp.ptr, None, res_input.len, PolyPtr.from_array(inp), PolyPtr.from_array(table),
