@@ -173,12 +173,31 @@ struct SrcHornerRev {  // (p_{n-1-k}, b)
     return horner_run(v, n - k0 - cnt, cnt, Fr::load(b));
   }
 };
-struct SinkKate {  // q_{n-1-k} = S of the fold of (p_j, b), j > n-1-k
-  PolyView q;
-  __device__ void operator()(uint64_t k, const HornerOp::T& excl, const HornerOp::T&, const HornerOp::T&) const {
-    st(q, q.len - 1 - k, excl.s);
+// Kate's rescan: thread t of tile b gets the fold of everything above its run
+// (the tile carry and the threads before it in mirrored order), whose S part is
+// q at the run's top index; below that, q_{i-1} = p_i + b q_i, one product per
+// element (the generic rescan would also carry the pair's b^count part).
+__global__ void __launch_bounds__(TH) k_kate_tile(PolyView p, PolyView q, const uint32_t* b, uint64_t n,
+                                                  const HornerOp::T* __restrict__ carry) {
+  using Op = RevOp<HornerOp>;
+  __shared__ HornerOp::T sh[TH];
+  const SrcHornerRev src{p, b};
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  HornerOp::T tot;
+  const HornerOp::T excl = block_exclusive<Op>(src.fold(k0, n), sh, &tot);
+  Fr s = Op::comb(carry[blockIdx.x], excl).s;
+  const Fr bb = Fr::load(b);
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k < n) {
+      const uint64_t i = n - 1 - k;
+      const Fr pi = ld(p, i);  // read before q_i is written: p and q may be one array
+      st(q, i, s);
+      if (j + 1 < PER) s = pi + bb * s;
+    }
   }
-};
+}
 
 // ---------------------------------------------------------------- batched inversion
 //
@@ -363,7 +382,12 @@ hipError_t kate_division(void* temp_buf, unsigned long* temp_buf_size, unsigned 
   using Op = RevOp<HornerOp>;
   auto* partial = reinterpret_cast<HornerOp::T*>(temp_buf);
   auto* carry = reinterpret_cast<HornerOp::T*>(reinterpret_cast<char*>(temp_buf) + part);
-  return run_scan<Op>(SrcHornerRev{vp, b}, SinkKate{vq}, n, partial, carry, (HornerOp::T*)nullptr, stream);
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_tile_fold<Op, SrcHornerRev>), dim3((unsigned)nb), dim3(TH), 0, stream, SrcHornerRev{vp, b}, n,
+                     partial);
+  hipLaunchKernelGGL((k_partials_scan<Op>), dim3(1), dim3(TH), 0, stream, partial, nb, carry, (HornerOp::T*)nullptr);
+  hipLaunchKernelGGL(k_kate_tile, dim3((unsigned)nb), dim3(TH), 0, stream, vp, vq, b, n, carry);
+  return hipGetLastError();
 }
 
 // target_i = target_0 * ... * target_i in place (poly.h:20)
