@@ -112,13 +112,6 @@ struct SrcProd {  // x_k
   PolyView v;
   __device__ ProdOp::T operator()(uint64_t k) const { return ProdOp::T{ld(v, k)}; }
 };
-struct SinkInclusive {  // x_k = prefix product through k
-  PolyView v;
-  __device__ void operator()(uint64_t k, const ProdOp::T&, const ProdOp::T&, const ProdOp::T& incl) const {
-    st(v, k, incl.v);
-  }
-};
-
 struct SrcNonZero {  // x_k, or 1 for x_k = 0
   PolyView v;
   __device__ ProdOp::T operator()(uint64_t k) const {
@@ -173,6 +166,31 @@ struct SrcHornerRev {  // (p_{n-1-k}, b)
     return horner_run(v, n - k0 - cnt, cnt, Fr::load(b));
   }
 };
+// scan_mul's rescan: every thread loads its 8 elements once, folds them for
+// the block scan, then runs the prefix product from its carry-in.
+__global__ void __launch_bounds__(TH) k_scan_mul_tile(PolyView v, uint64_t n, const ProdOp::T* __restrict__ carry) {
+  __shared__ ProdOp::T sh[TH];
+  const uint64_t k0 = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * PER;
+  Fr x[PER];
+  Fr t = Fr::one();
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    x[j] = k < n ? ld(v, k) : Fr::one();
+    t = j ? t * x[j] : x[j];
+  }
+  ProdOp::T tot;
+  Fr acc = carry[blockIdx.x].v * block_exclusive<ProdOp>(ProdOp::T{t}, sh, &tot).v;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const uint64_t k = k0 + j;
+    if (k < n) {
+      acc = acc * x[j];
+      st(v, k, acc);
+    }
+  }
+}
+
 // Kate's rescan: thread t of tile b gets the fold of everything above its run
 // (the tile carry and the threads before it in mirrored order), whose S part is
 // q at the run's top index; below that, q_{i-1} = p_i + b q_i, one product per
@@ -405,7 +423,13 @@ hipError_t scan_mul(void* temp_buffer, unsigned long* buffer_size, PolyPtr targe
   if (v.ptr == nullptr) return hipErrorInvalidValue;
   auto* partial = reinterpret_cast<ProdOp::T*>(temp_buffer);
   auto* carry = reinterpret_cast<ProdOp::T*>(reinterpret_cast<char*>(temp_buffer) + part);
-  return run_scan<ProdOp>(SrcProd{v}, SinkInclusive{v}, v.len, partial, carry, (ProdOp::T*)nullptr, stream);
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_tile_fold<ProdOp, SrcProd>), dim3((unsigned)nb), dim3(TH), 0, stream, SrcProd{v}, v.len,
+                     partial);
+  hipLaunchKernelGGL((k_partials_scan<ProdOp>), dim3(1), dim3(TH), 0, stream, partial, nb, carry,
+                     (ProdOp::T*)nullptr);
+  hipLaunchKernelGGL(k_scan_mul_tile, dim3((unsigned)nb), dim3(TH), 0, stream, v, v.len, carry);
+  return hipGetLastError();
 }
 
 // poly_i = poly_i^-1 in place, zeros kept (poly.h:22)
