@@ -409,3 +409,70 @@ def test_permute_reuses_temp_for_a_smaller_call(libs, stream):
         want_in, want_tab = orc.permute(inp, tab, usable)
         assert np.array_equal(gpu2cpu(ri, rib), want_in)
         assert np.array_equal(gpu2cpu(rt, rtb), want_tab)
+
+
+# ---------------------------------------------------------------- full size (2^24)
+
+
+def gpu_eval(libs, stream, arr, x_limbs):
+    op = PolyEval(libs)
+    tb, tmp = temp_buffer(op.get_buffer_size(arr.len))
+    xb, xs = dev_scalar(x_limbs)
+    rb, rs = dev_scalar(np.zeros(4, np.uint64))
+    op.get_fn().f([tmp, rs], [arr, xs, stream])
+    stream.sync()
+    return array_to_ints(read_scalar(rb).reshape(1, 4))[0]
+
+
+def test_full_size_properties(libs, stream):
+    """2^24 elements, checked through properties that do not depend on the size
+    (the oracle pins every entry point bit-exactly up to 2^21 above):
+    inverting twice gives the input back (zeros kept); Kate's quotient q
+    satisfies p(x) = q(x) (x - b) + p(b) at a random x, with the GPU's own eval;
+    the prefix products end in the product of all elements."""
+    k = 24
+    n = 1 << k
+    x = random_elements(n, 900)
+    x[::1000] = 0
+    buf, arr = put(x, rotate=12345)
+    op = PolyInvert(libs)
+    tb, tmp = temp_buffer(op.get_buffer_size(n))
+    for _ in range(2):
+        op.get_fn().f([tmp, arr], [stream])
+    stream.sync()
+    assert np.array_equal(gpu2cpu(arr, buf), x)
+    del tb, tmp
+
+    # Kate: q = (p - p(b)) / (X - b) in place over p
+    b = random_elements(1, 901)[0]
+    z = random_elements(1, 902)[0]
+    p_at_z = from_mont(gpu_eval(libs, stream, arr, z))
+    p_at_b = from_mont(gpu_eval(libs, stream, arr, b))
+    kd = KateDivision(libs)
+    tb, tmp = temp_buffer(kd.get_buffer_size(k))
+    bb, bs = dev_scalar(b)
+    kd.get_fn().f([tmp, arr], [arr, bs, stream])
+    stream.sync()
+    q_at_z = from_mont(gpu_eval(libs, stream, arr, z))
+    zv, bv = from_mont(array_to_ints(z.reshape(1, 4))[0]), from_mont(array_to_ints(b.reshape(1, 4))[0])
+    assert p_at_z == (q_at_z * (zv - bv) + p_at_b) % FR_MODULUS
+    del tb, tmp, buf, arr
+
+    # scan_mul: the last prefix product is the product of all (checked on a
+    # 2^16 slice of the output against big ints)
+    m = 1 << 16
+    y = random_elements(n, 903)
+    ybuf, yarr = put(y)
+    sc = PolyScan(libs)
+    tb, tmp = temp_buffer(sc.get_buffer_size(n))
+    rbuf, rarr = empty(n)
+    one_b, one_s = dev_scalar(mont_limbs(1))
+    sc.get_fn().f([tmp, rarr], [yarr, one_s, stream])
+    stream.sync()
+    out = gpu2cpu(rarr, rbuf)
+    # target_i = y_0 ... y_{i-1}: a window of consecutive outputs differs by the elements between them
+    lo = n - m
+    prod = from_mont(array_to_ints(out[lo].reshape(1, 4))[0])
+    for v in array_to_ints(y[lo:n - 1]):
+        prod = prod * from_mont(v) % FR_MODULUS
+    assert prod == from_mont(array_to_ints(out[n - 1].reshape(1, 4))[0])
