@@ -38,6 +38,8 @@ namespace perm {
 constexpr int STH = 256;           // threads of a sort workgroup
 constexpr int SPER = 8;            // keys per thread
 constexpr int STILE = STH * SPER;  // 2048 keys: 64 KiB of LDS
+constexpr int MTH = 512;           // threads of a merge-pass workgroup (the same 2048-key tile)
+constexpr int MPER = STILE / MTH;  // outputs per thread
 constexpr uint32_t START = 0xffffffffu;
 
 // a < b as 256-bit integers: the borrow out of a - b.
@@ -88,14 +90,14 @@ __device__ __forceinline__ uint32_t merge_path(const GA& A, uint32_t la, const G
   return lo;
 }
 
-// m (<= SPER) outputs of merge(A, B) from split (i, j), ties going to A.
-template <class GA, class GB>
+// m (<= N) outputs of merge(A, B) from split (i, j), ties going to A.
+template <int N, class GA, class GB>
 __device__ __forceinline__ void merge_run(const GA& A, uint32_t la, const GB& B, uint32_t lb, uint32_t i, uint32_t j,
-                                          uint32_t m, Fr (&out)[SPER]) {
+                                          uint32_t m, Fr (&out)[N]) {
   Fr a = i < la ? A(i) : Fr::zero();
   Fr b = j < lb ? B(j) : Fr::zero();
 #pragma unroll
-  for (int q = 0; q < SPER; q++) {
+  for (int q = 0; q < N; q++) {
     if ((uint32_t)q < m) {
       const bool take_a = i < la && (j >= lb || !key_lt(b, a));
       if (take_a) {
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(STH) k_tile_sort(Fr* k0, Fr* k1, uint32_t n) {
     const auto B = [&](uint32_t j) { return lds_get(s, pb + L + j); };
     const uint32_t i = merge_path(A, L, B, L, d);
     Fr out[SPER];
-    merge_run(A, L, B, L, i, d - i, SPER, out);
+    merge_run<SPER>(A, L, B, L, i, d - i, SPER, out);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPER; q++) lds_put(s, d0 + q, out[q]);
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(TH) k_merge_split(const Fr* in0, const Fr* in1
 
 // 2c. Runs of L sorted keys merged pairwise into runs of 2L: workgroup b
 // writes outputs [b*STILE, min((b+1)*STILE, n)) from the splits of 2b.
-__global__ void __launch_bounds__(STH) k_merge_pass(const Fr* in0, Fr* out0, const Fr* in1, Fr* out1, uint32_t n,
+__global__ void __launch_bounds__(MTH) k_merge_pass(const Fr* in0, Fr* out0, const Fr* in1, Fr* out1, uint32_t n,
                                                      uint32_t L, const uint32_t* split) {
   __shared__ uint4 s[2 * STILE];
   const Fr* in = blockIdx.y ? in1 : in0;
@@ -222,29 +224,29 @@ __global__ void __launch_bounds__(STH) k_merge_pass(const Fr* in0, Fr* out0, con
   const uint32_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t na = i1 - i0, nb = j1 - j0;  // na + nb = o1 - o0 <= STILE
   const uint32_t tot = na + nb;
-  for (uint32_t c = threadIdx.x; c < 2 * tot; c += STH) {
+  for (uint32_t c = threadIdx.x; c < 2 * tot; c += MTH) {
     const uint32_t q = c >> 1;
     const uint4* src = q < na ? reinterpret_cast<const uint4*>(A + i0 + q)
                               : reinterpret_cast<const uint4*>(B + j0 + (q - na));
     s[swz(c)] = src[c & 1];
   }
   __syncthreads();
-  const uint32_t d = threadIdx.x * SPER;
-  const uint32_t m = d < tot ? (tot - d < (uint32_t)SPER ? tot - d : (uint32_t)SPER) : 0;
-  Fr res[SPER];
+  const uint32_t d = threadIdx.x * MPER;
+  const uint32_t m = d < tot ? (tot - d < (uint32_t)MPER ? tot - d : (uint32_t)MPER) : 0;
+  Fr res[MPER];
   if (m) {
     const auto LA = [&](uint32_t i) { return lds_get(s, i); };
     const auto LB = [&](uint32_t j) { return lds_get(s, na + j); };
     const uint32_t i = merge_path(LA, na, LB, nb, d);
-    merge_run(LA, na, LB, nb, i, d - i, m, res);
+    merge_run<MPER>(LA, na, LB, nb, i, d - i, m, res);
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < SPER; q++)
+  for (int q = 0; q < MPER; q++)
     if ((uint32_t)q < m) lds_put(s, d + q, res[q]);
   __syncthreads();
   uint4* dst = reinterpret_cast<uint4*>(out + o0);
-  for (uint32_t c = threadIdx.x; c < 2 * tot; c += STH) dst[c] = s[swz(c)];
+  for (uint32_t c = threadIdx.x; c < 2 * tot; c += MTH) dst[c] = s[swz(c)];
 }
 
 __global__ void __launch_bounds__(TH) k_zero(uint32_t* x, uint32_t n) {
@@ -382,7 +384,7 @@ hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long 
   for (uint64_t L = STILE; L < n; L <<= 1) {
     hipLaunchKernelGGL(k_merge_split, dim3((tiles + TH - 1) / TH, 2), dim3(TH), 0, stream, ka[cur], ks[cur], n,
                        (uint32_t)L, tiles, split);
-    hipLaunchKernelGGL(k_merge_pass, dim3(tiles, 2), dim3(STH), 0, stream, ka[cur], ka[cur ^ 1], ks[cur],
+    hipLaunchKernelGGL(k_merge_pass, dim3(tiles, 2), dim3(MTH), 0, stream, ka[cur], ka[cur ^ 1], ks[cur],
                        ks[cur ^ 1], n, (uint32_t)L, split);
     cur ^= 1;
   }
