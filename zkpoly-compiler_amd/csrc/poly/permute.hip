@@ -38,7 +38,7 @@ namespace perm {
 constexpr int STH = 256;           // threads of a sort workgroup
 constexpr int SPER = 8;            // keys per thread
 constexpr int STILE = STH * SPER;  // 2048 keys: 64 KiB of LDS
-constexpr int MTH = 512;           // threads of a merge-pass workgroup (the same 2048-key tile)
+constexpr int MTH = 1024;          // threads of a merge-pass workgroup (the same 2048-key tile)
 constexpr int MPER = STILE / MTH;  // outputs per thread
 constexpr uint32_t START = 0xffffffffu;
 
