@@ -16,8 +16,8 @@
 // Steps, all asynchronous on `stream`, allocating nothing:
 //  1. canonical keys of A and S into two key buffers (one launch for both);
 //  2. sort, the same launches serving both arrays: every workgroup sorts a tile
-//     of 2048 keys in LDS (8 per thread by a sorting network, then merges of
-//     runs 8 .. 1024); merge passes then double the sorted runs through HBM:
+//     of 2048 keys in LDS (4 per thread by a sorting network, then merges of
+//     runs 4 .. 1024); merge passes then double the sorted runs through HBM:
 //     one thread per output tile finds the tile's merge-path split, then each
 //     workgroup merges its 2048 outputs staged in LDS.  Keys are whole 256-bit
 //     values compared by a borrow chain, so the cost does not depend on how
@@ -35,9 +35,9 @@ namespace zkp {
 namespace poly {
 namespace perm {
 
-constexpr int STH = 256;           // threads of a sort workgroup
-constexpr int SPER = 8;            // keys per thread
-constexpr int STILE = STH * SPER;  // 2048 keys: 64 KiB of LDS
+constexpr int STILE = 2048;        // keys per LDS tile: 64 KiB
+constexpr int TTH = 512;           // threads of a tile-sort workgroup
+constexpr int TPER = STILE / TTH;  // keys per thread of the tile sort (a 4-key network, then merges)
 constexpr int MTH = 1024;          // threads of a merge-pass workgroup (the same 2048-key tile)
 constexpr int MPER = STILE / MTH;  // outputs per thread
 constexpr uint32_t START = 0xffffffffu;
@@ -123,48 +123,46 @@ __global__ void __launch_bounds__(TH) k_to_keys(PolyView a, PolyView t, Fr* ka, 
 }
 
 // 2a. Sorts keys[b*STILE, min((b+1)*STILE, n)) in place; the tile's missing
-// tail is padded with 2^256 - 1, above every canonical value.
-__global__ void __launch_bounds__(STH) k_tile_sort(Fr* k0, Fr* k1, uint32_t n) {
+// tail is padded with 2^256 - 1, above every canonical value.  Each of the TTH
+// threads sorts TPER keys in registers, then runs of TPER .. STILE / 2 are
+// merged pairwise in LDS (short serial chains and many waves: the merges wait
+// on LDS latency, not on VALU).
+__global__ void __launch_bounds__(TTH) k_tile_sort(Fr* k0, Fr* k1, uint32_t n) {
   __shared__ uint4 s[2 * STILE];
   Fr* keys = pick(k0, k1);
   const uint32_t base = blockIdx.x * STILE;
   const uint32_t cnt = n - base < (uint32_t)STILE ? n - base : (uint32_t)STILE;
   const uint4* src = reinterpret_cast<const uint4*>(keys + base);
-  for (uint32_t c = threadIdx.x; c < 2u * STILE; c += STH)
+  for (uint32_t c = threadIdx.x; c < 2u * STILE; c += TTH)
     s[swz(c)] = (c >> 1) < cnt ? src[c] : make_uint4(~0u, ~0u, ~0u, ~0u);
   __syncthreads();
-  const uint32_t d0 = threadIdx.x * SPER;
+  const uint32_t d0 = threadIdx.x * TPER;
   {
-    Fr x[SPER];
+    static_assert(TPER == 4, "the register network below sorts four keys");
+    Fr x[TPER];
 #pragma unroll
-    for (int q = 0; q < SPER; q++) x[q] = lds_get(s, d0 + q);
-    // Batcher's odd-even merge network for 8 keys (19 compare-exchanges)
-    cas(x[0], x[1]), cas(x[2], x[3]), cas(x[4], x[5]), cas(x[6], x[7]);
-    cas(x[0], x[2]), cas(x[1], x[3]), cas(x[4], x[6]), cas(x[5], x[7]);
-    cas(x[1], x[2]), cas(x[5], x[6]);
-    cas(x[0], x[4]), cas(x[1], x[5]), cas(x[2], x[6]), cas(x[3], x[7]);
-    cas(x[2], x[4]), cas(x[3], x[5]);
-    cas(x[1], x[2]), cas(x[3], x[4]), cas(x[5], x[6]);
+    for (int q = 0; q < TPER; q++) x[q] = lds_get(s, d0 + q);
+    cas(x[0], x[1]), cas(x[2], x[3]), cas(x[0], x[2]), cas(x[1], x[3]), cas(x[1], x[2]);
 #pragma unroll
-    for (int q = 0; q < SPER; q++) lds_put(s, d0 + q, x[q]);  // the thread's own slots
+    for (int q = 0; q < TPER; q++) lds_put(s, d0 + q, x[q]);  // the thread's own slots
   }
   __syncthreads();
-  for (uint32_t L = SPER; L < (uint32_t)STILE; L <<= 1) {
-    // the thread's outputs [d0, d0 + 8) lie in one pair of runs (2L >= 16)
+  for (uint32_t L = TPER; L < (uint32_t)STILE; L <<= 1) {
+    // the thread's outputs [d0, d0 + TPER) lie in one pair of runs (2L > TPER)
     const uint32_t pb = d0 & ~(2 * L - 1);
     const uint32_t d = d0 - pb;
     const auto A = [&](uint32_t i) { return lds_get(s, pb + i); };
     const auto B = [&](uint32_t j) { return lds_get(s, pb + L + j); };
     const uint32_t i = merge_path(A, L, B, L, d);
-    Fr out[SPER];
-    merge_run<SPER>(A, L, B, L, i, d - i, SPER, out);
+    Fr out[TPER];
+    merge_run<TPER>(A, L, B, L, i, d - i, TPER, out);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < SPER; q++) lds_put(s, d0 + q, out[q]);
+    for (int q = 0; q < TPER; q++) lds_put(s, d0 + q, out[q]);
     __syncthreads();
   }
   uint4* dst = reinterpret_cast<uint4*>(keys + base);
-  for (uint32_t c = threadIdx.x; c < 2u * cnt; c += STH) dst[c] = s[swz(c)];
+  for (uint32_t c = threadIdx.x; c < 2u * cnt; c += TTH) dst[c] = s[swz(c)];
 }
 
 // The pair of runs of L (the last run may be short, or missing) holding
@@ -379,7 +377,7 @@ hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long 
   const unsigned g = grid_for(n);
   hipLaunchKernelGGL(k_to_keys, dim3(g, 2), dim3(TH), 0, stream, vi, vt, ka[0], ks[0], n);
   const unsigned tiles = (n + STILE - 1) / STILE;
-  hipLaunchKernelGGL(k_tile_sort, dim3(tiles, 2), dim3(STH), 0, stream, ka[0], ks[0], n);
+  hipLaunchKernelGGL(k_tile_sort, dim3(tiles, 2), dim3(TTH), 0, stream, ka[0], ks[0], n);
   int cur = 0;
   for (uint64_t L = STILE; L < n; L <<= 1) {
     hipLaunchKernelGGL(k_merge_split, dim3((tiles + TH - 1) / TH, 2), dim3(TH), 0, stream, ka[cur], ks[cur], n,
