@@ -25,9 +25,10 @@
 //     expressions and negatives alike);
 //  3. every run start of A' finds its lower bound in S' and marks that table
 //     slot used; A' is written out in Montgomery form;
-//  4. one scan over rows of (run starts, unused slots) counts compacts the
-//     unused slots in order into the leftover list, writes S' at run starts,
-//     and gives every other row its rank among the non-starts;
+//  4. one scan over rows of (run starts, unused slots) counts (tile folds, then
+//     a rescan by wave ballots) compacts the unused slots in order into the
+//     leftover list, writes S' at run starts, and gives every other row its
+//     rank among the non-starts;
 //  5. the row of rank r takes leftover r.
 #include "scan.hpp"
 
@@ -310,6 +311,47 @@ struct SinkCounts {
   }
 };
 
+// 4b. The rescan of a tile for 0/1 flags, one element per lane: each row of TH
+// consecutive elements gets its exclusive counts from wave ballots (the set
+// lanes below a lane, by mbcnt) and the totals of the waves before it (LDS);
+// rows follow one another from the tile's carry.  Loads and stores are
+// coalesced, with no per-thread serial chain.
+__global__ void __launch_bounds__(TH) k_count_tile(const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ used, uint64_t n,
+                                                   const CountOp::T* __restrict__ carry, SinkCounts sink) {
+  constexpr uint32_t WAVES = TH / 64;
+  __shared__ uint32_t wf[WAVES], wu[WAVES];
+  const uint32_t wave = threadIdx.x >> 6;
+  CountOp::T run = carry[blockIdx.x];
+  for (uint32_t r = 0; r < (uint32_t)PER; r++) {
+    const uint64_t k = (uint64_t)blockIdx.x * TILE + (uint64_t)r * TH + threadIdx.x;
+    const bool in = k < n;
+    const uint32_t f = in ? start[k] : 0u;
+    const uint32_t u = in ? 1u - used[k] : 0u;
+    const unsigned long long mf = __ballot(f != 0), mu = __ballot(u != 0);
+    const uint32_t bf = __builtin_amdgcn_mbcnt_hi((uint32_t)(mf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mf, 0u));
+    const uint32_t bu = __builtin_amdgcn_mbcnt_hi((uint32_t)(mu >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mu, 0u));
+    if ((threadIdx.x & 63) == 0) {
+      wf[wave] = (uint32_t)__popcll(mf);
+      wu[wave] = (uint32_t)__popcll(mu);
+    }
+    __syncthreads();
+    CountOp::T excl = run;
+    uint32_t rf = 0, ru = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < WAVES; w++) {
+      if (w < wave) excl.f += wf[w], excl.u += wu[w];
+      rf += wf[w], ru += wu[w];
+    }
+    __syncthreads();  // wf / wu are rewritten by the next row
+    excl.f += bf;
+    excl.u += bu;
+    if (in) sink(k, excl, CountOp::T{f, u}, CountOp::T{excl.f + f, excl.u + u});
+    run.f += rf;
+    run.u += ru;
+  }
+}
+
 // 5. The non-start row of rank r takes leftover r (r < #non-starts <= #unused).
 __global__ void __launch_bounds__(TH) k_fill_back(const uint32_t* rank, const Fr* leftover, uint32_t n,
                                                   PolyView res_table) {
@@ -391,9 +433,13 @@ hipError_t permute(void* temp_buffer, unsigned long* buffer_size, unsigned long 
   Fr* leftover = ks[cur ^ 1];
   hipLaunchKernelGGL(k_zero, dim3(g), dim3(TH), 0, stream, used, n);
   hipLaunchKernelGGL(k_lookup, dim3(g), dim3(TH), 0, stream, sa, ss, n, used, start, ri);
-  hipError_t err = run_scan<CountOp>(SrcCounts{start, used}, SinkCounts{sa, ss, leftover, start, rt}, n, partial,
-                                     carry, (CountOp::T*)nullptr, stream);
-  if (err != hipSuccess) return err;
+  const uint64_t cnb = tiles_of(n);
+  hipLaunchKernelGGL((k_tile_fold<CountOp, SrcCounts>), dim3((unsigned)cnb), dim3(TH), 0, stream,
+                     SrcCounts{start, used}, (uint64_t)n, partial);
+  hipLaunchKernelGGL((k_partials_scan<CountOp>), dim3(1), dim3(TH), 0, stream, partial, cnb, carry,
+                     (CountOp::T*)nullptr);
+  hipLaunchKernelGGL(k_count_tile, dim3((unsigned)cnb), dim3(TH), 0, stream, start, used, (uint64_t)n, carry,
+                     SinkCounts{sa, ss, leftover, start, rt});
   hipLaunchKernelGGL(k_fill_back, dim3(g), dim3(TH), 0, stream, start, leftover, n, rt);
   return hipGetLastError();
 }
