@@ -1,21 +1,21 @@
 """HBM traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated on a plain copy.
 
-`python -m zkpoly_amd.traffic run` launches a streaming copy of COPY_BYTES (read
+`python3 rocprof_tools.py run` launches a streaming copy of COPY_BYTES (read
 COPY_BYTES, write COPY_BYTES) and then `ntts` forward NTTs of 2^log_n through
 ssip_ntt; run it once under `rocprofv3 --pmc FETCH_SIZE` and once under
 `rocprofv3 --pmc WRITE_SIZE` (one counter set per run).
 
-`python -m zkpoly_amd.traffic summarize FETCH.csv WRITE.csv OUT.json OUT.md`
+`python3 rocprof_tools.py summarize FETCH.csv WRITE.csv OUT.json OUT.md`
 averages each counter per kernel, scales it by (known copy bytes) / (what the
 counter reads for the copy), and writes bytes per launch.  A kernel whose
 traffic is well above its algorithmic bytes re-reads data.
 
-`python -m zkpoly_amd.traffic valu NTT.csv MSM.csv OUT.json OUT.md` averages
+`python3 rocprof_tools.py valu NTT.csv MSM.csv OUT.json OUT.md` averages
 rocprofv3's derived VALUBusy (percent of time the vector ALUs issue) and
 VALUUtilization (percent of active lanes) per kernel, from runs of
 `rocprofv3 --pmc VALUBusy VALUUtilization` over bench.py's NTT and MSM legs.
 
-`python -m zkpoly_amd.traffic trace DIR TITLE COMMAND OUT.md [KERNEL N]` writes
+`python3 rocprof_tools.py trace DIR TITLE COMMAND OUT.md [KERNEL N]` writes
 the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (DIR holds its
 run_kernel_stats.csv) as markdown, with the command that made it, and the
 average of KERNEL's last N dispatches (bench.py's timed region).
@@ -24,8 +24,12 @@ from __future__ import annotations
 
 import csv
 import json
+import os
 import sys
 from collections import defaultdict
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zkpoly-compiler_amd"))
 
 COPY_BYTES = 512 << 20
 COPY_KERNEL = "copy_u4_kernel"
@@ -34,10 +38,10 @@ COPY_KERNEL = "copy_u4_kernel"
 def run(log_n: int = 24, ntts: int = 3) -> None:
     import ctypes
 
-    from . import hip
-    from .field import mont_limbs, omega, random_elements
-    from .ntt import gen_powers, load_lib
-    from .runtime import Libs, PolyPtr
+    from zkpoly_amd import hip
+    from zkpoly_amd.field import mont_limbs, omega, random_elements
+    from zkpoly_amd.ntt import gen_powers, load_lib
+    from zkpoly_amd.runtime import Libs, PolyPtr
 
     libs = Libs()
     stream = hip.Stream(0)
@@ -97,8 +101,8 @@ def summarize(fetch_csv: str, write_csv: str, out_json: str, out_md: str, log_n:
         f.write("cd /tmp && export TMPDIR=/tmp && export PYTHONPATH=$REPO/zkpoly-compiler_amd\n")
         for c, d in (("FETCH_SIZE", "pmc_fetch"), ("WRITE_SIZE", "pmc_write")):
             f.write(f"rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/{d} -o run -- "
-                    "python3 -u -m zkpoly_amd.traffic run\n")
-        f.write("python3 -m zkpoly_amd.traffic summarize $REPO/gpurun_out/pmc_fetch/run_counter_collection.csv "
+                    "python3 -u rocprof_tools.py run\n")
+        f.write("python3 rocprof_tools.py summarize $REPO/gpurun_out/pmc_fetch/run_counter_collection.csv "
                 "$REPO/gpurun_out/pmc_write/run_counter_collection.csv profiles/pmc_traffic.json "
                 "profiles/pmc_traffic_ntt_2e24.md\n```\n\n")
         f.write(f"Calibration: the copy kernel moves {COPY_BYTES} B each way; the counters read "
@@ -132,7 +136,7 @@ def summarize_valu(ntt_csv: str, msm_csv: str, out_json: str, out_md: str) -> di
                 "-- python3 -u $REPO/bench.py --quick --no-cpu --steps 3 --warmup 1\n")
         f.write("rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d $REPO/gpurun_out/pmc_valu_msm -o run "
                 "-- python3 -u $REPO/bench.py --msm-only --no-cpu --steps 2 --warmup 1\n")
-        f.write("python3 -m zkpoly_amd.traffic valu $REPO/gpurun_out/pmc_valu_ntt/run_counter_collection.csv "
+        f.write("python3 rocprof_tools.py valu $REPO/gpurun_out/pmc_valu_ntt/run_counter_collection.csv "
                 "$REPO/gpurun_out/pmc_valu_msm/run_counter_collection.csv profiles/pmc_valu.json "
                 "profiles/pmc_valu.md\n```\n\n")
         f.write("VALUBusy: percent of the kernel's time the vector ALUs issue instructions; VALUUtilization: percent "

@@ -426,10 +426,11 @@ def gpu_eval(libs, stream, arr, x_limbs):
 
 def test_full_size_properties(libs, stream):
     """2^24 elements, checked through properties that do not depend on the size
-    (the oracle pins every entry point bit-exactly up to 2^21 above):
-    inverting twice gives the input back (zeros kept); Kate's quotient q
-    satisfies p(x) = q(x) (x - b) + p(b) at a random x, with the GPU's own eval;
-    the prefix products end in the product of all elements."""
+    (the oracle pins every entry point bit-exactly up to 2^21 above): after one
+    inversion x_i * y_i = 1 on 4096 sampled rows and every zero is still zero,
+    and inverting again gives the input back; Kate's quotient q satisfies
+    p(x) = q(x) (x - b) + p(b) at a random x, with the GPU's own eval; the
+    prefix products end in the product of all elements."""
     k = 24
     n = 1 << k
     x = random_elements(n, 900)
@@ -437,8 +438,15 @@ def test_full_size_properties(libs, stream):
     buf, arr = put(x, rotate=12345)
     op = PolyInvert(libs)
     tb, tmp = temp_buffer(op.get_buffer_size(n))
-    for _ in range(2):
-        op.get_fn().f([tmp, arr], [stream])
+    op.get_fn().f([tmp, arr], [stream])
+    stream.sync()
+    y = gpu2cpu(arr, buf)
+    zero = ~x.any(axis=1)
+    assert not y[zero].any(), "zeros must stay zero"
+    rows = np.random.default_rng(904).choice(np.flatnonzero(~zero), 4096, replace=False)
+    prod = orc.vec("orc_vec_mul", FR, np.ascontiguousarray(x[rows]), np.ascontiguousarray(y[rows]))
+    assert np.array_equal(prod, np.broadcast_to(mont_limbs(1), prod.shape)), "x * x^-1 != 1"
+    op.get_fn().f([tmp, arr], [stream])
     stream.sync()
     assert np.array_equal(gpu2cpu(arr, buf), x)
     del tb, tmp
@@ -476,3 +484,32 @@ def test_full_size_properties(libs, stream):
     for v in array_to_ints(y[lo:n - 1]):
         prod = prod * from_mont(v) % FR_MODULUS
     assert prod == from_mont(array_to_ints(out[n - 1].reshape(1, 4))[0])
+
+
+def test_permute_full_size_properties(libs, stream):
+    """2^24 rows with values below 2^20 (the oracle pins permute bit-exactly up
+    to 2^21 + 3 rows above).  These properties fix the result completely:
+    A' is the input in ascending canonical order; at every run start of A' the
+    table column holds the run's value; the other rows hold the table entries
+    left over after the run starts, ascending, in row order."""
+    n = 1 << 24
+    inp, tab = lookup_case(n, n, 905, 1 << 20)
+    got_in, got_tab = run_permute(libs, stream, inp, tab, n)
+    one = np.zeros((n, 4), dtype=np.uint64)
+    one[:, 0] = 1
+
+    def small(a):  # Montgomery -> canonical; the values fit in limb 0
+        c = orc.vec("orc_vec_mul", FR, np.ascontiguousarray(a), one)
+        assert not c[:, 1:].any()
+        return c[:, 0]
+
+    a, s, i_c, t_c = small(got_in), small(got_tab), small(inp), small(tab)
+    assert np.array_equal(a, np.sort(i_c)), "A' is the sorted input"
+    start = np.ones(n, dtype=bool)
+    start[1:] = a[1:] != a[:-1]
+    assert np.array_equal(s[start], a[start]), "run starts take their own value"
+    used = np.bincount(a[start].astype(np.int64), minlength=1 << 20)
+    have = np.bincount(t_c.astype(np.int64), minlength=1 << 20)
+    assert (have >= used).all()
+    left = np.repeat(np.arange(1 << 20, dtype=np.uint64), have - used)
+    assert np.array_equal(s[~start], left), "leftovers ascending in row order"

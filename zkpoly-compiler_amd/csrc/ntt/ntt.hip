@@ -37,6 +37,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -1069,15 +1070,20 @@ struct PlanV2 {
   PassPlan pass[48];
 };
 
-// per log_n minimum column exponent for v2 passes (runs of >= 2^c elements);
-// set by the autotuner through zkp_ntt_set_tuning.  The default 1 (fewest,
-// deepest passes) measured fastest at 2^20..2^24 with 512-thread passes.
-static uint32_t g_min_col_log[41] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
-                                     1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+// Tuning state, read by every launch and written by the autotuner hooks
+// (zkp_ntt_set_variant / zkp_ntt_set_tuning).  Both are atomics and a launch
+// reads each once, so a setter that races with launches on other threads
+// changes which plan the next launch picks but never mixes two plans.
+//
+// per log_n minimum column exponent for v2 passes (runs of >= 2^c elements).
+// The default 1 (fewest, deepest passes) measured fastest at 2^20..2^24 with
+// 512-thread passes.
+static std::atomic<uint32_t> g_min_col_log[41] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                                  1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
 // 0: persistent head + radix-4 passes; 1: plain bit reversal + radix-2 passes;
 // 2: as 0 with one tile pair per head workgroup; 3: as 0 with 256-thread
 // passes (batch plans treat 2 and 3 as 0)
-static int g_variant = 0;
+static std::atomic<int> g_variant{0};
 
 static void plan_v2(uint32_t log_n, PlanV2* p) {
   uint32_t t = log_n / 2;
@@ -1085,7 +1091,7 @@ static void plan_v2(uint32_t log_n, PlanV2* p) {
   p->head_t = t;
   p->np = 0;
   uint32_t s0 = t;
-  uint32_t min_c = g_min_col_log[log_n < 41 ? log_n : 40];
+  uint32_t min_c = g_min_col_log[log_n < 41 ? log_n : 40].load(std::memory_order_relaxed);
   if (min_c > TILE_LOG - 1) min_c = TILE_LOG - 1;
   const uint32_t max_d = TILE_LOG - min_c;
   while (s0 < log_n) {
@@ -1163,7 +1169,8 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
     err = hipEventRecord(events[ev++], stream);
     if (err != hipSuccess) return err;
   }
-  if (g_variant == 1) {
+  const int variant = g_variant.load(std::memory_order_relaxed);
+  if (variant == 1) {
     hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
                        (uint32_t)log_len);
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
@@ -1181,7 +1188,7 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
       hipLaunchKernelGGL(bitrev_kernel, dim3(grid_for(v.len, THREADS, 65536)), dim3(THREADS), 0, stream, v,
                          (uint32_t)log_len);
     } else {
-      if (g_variant == 2) {
+      if (variant == 2) {
         hipLaunchKernelGGL(bitrev_head, dim3((unsigned)(1ull << (log_len - 2 * plan.head_t))), dim3(THREADS), 0,
                            stream, v, twiddle, (uint32_t)log_len, plan.head_t);
       } else {
@@ -1193,7 +1200,7 @@ static hipError_t launch_ntt(PolyPtr x, const unsigned int* twiddle, unsigned in
     if ((err = hipGetLastError()) != hipSuccess || (err = mark(0)) != hipSuccess) return err;
     for (int p = 0; p < plan.np; p++) {
       const PassPlan& pp = plan.pass[p];
-      if (g_variant == 3)
+      if (variant == 3)
         hipLaunchKernelGGL(dit_pass_r4_narrow, dim3((unsigned)pp.tiles), dim3(THREADS), 0, stream, v, twiddle,
                            (uint32_t)log_len, pp.s0, pp.d, pp.c_log, (uint32_t)(p + 1 == plan.np));
       else
@@ -1262,13 +1269,11 @@ hipError_t zkp_ntt_with_events(PolyPtr x, const unsigned int* twiddle, unsigned 
 // Transform variant for all later calls and batch plans in this process:
 // 0 = fused bit reversal + radix-4 passes (default), 1 = separate bit reversal
 // + radix-2 passes.  Returns the previous variant, or -1 for an unknown one.
-// Not thread-safe against concurrent launches.  No counterpart in the
-// reference ABI (tuning hook).
+// Safe to call while other threads launch transforms: each launch reads the
+// variant once.  No counterpart in the reference ABI (tuning hook).
 int zkp_ntt_set_variant(int variant) {
   if (variant < 0 || variant > 3) return -1;
-  const int old = zkp::ntt::g_variant;
-  zkp::ntt::g_variant = variant;
-  return old;
+  return zkp::ntt::g_variant.exchange(variant);
 }
 
 // Tuning of transforms of 2^log_n (variant 0): every pass reads runs of at
@@ -1277,7 +1282,7 @@ int zkp_ntt_set_variant(int variant) {
 // reference ABI (autotuner hook).
 int zkp_ntt_set_tuning(unsigned int log_n, unsigned int min_col_log) {
   if (log_n > 40 || min_col_log > (unsigned)zkp::ntt::TILE_LOG - 1) return -1;
-  zkp::ntt::g_min_col_log[log_n] = min_col_log;
+  zkp::ntt::g_min_col_log[log_n].store(min_col_log, std::memory_order_relaxed);
   return 0;
 }
 
@@ -1285,7 +1290,7 @@ int zkp_ntt_set_tuning(unsigned int log_n, unsigned int min_col_log) {
 // variant and tuning (0 for log_n == 0 or > 40).
 int zkp_ntt_kernel_count(unsigned int log_n) {
   if (log_n == 0 || log_n > 40) return 0;
-  if (zkp::ntt::g_variant == 1) {
+  if (zkp::ntt::g_variant.load() == 1) {
     zkp::ntt::PassPlan plan[48];
     return 1 + zkp::ntt::plan_passes(log_n, plan);
   }
@@ -1443,7 +1448,7 @@ hipError_t zkp_ntt_batch_create(const PolyPtr* xs, const unsigned int* const* tw
     return hipErrorInvalidValue;
   BatchPlan* plan = new BatchPlan();
   plan->count = count;
-  plan->variant = g_variant;
+  plan->variant = g_variant.load();
   std::vector<BatchInstance> inst(count);
   std::vector<uint64_t> elem_prefix(count + 1, 0);
   std::vector<std::vector<PassPlan>> plans(count);

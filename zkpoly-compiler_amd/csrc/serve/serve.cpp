@@ -124,6 +124,7 @@ namespace {
 
 struct Engine {
   ZkpServeConfig cfg{};
+  int device = 0;  // the device current at zkp_serve_create; workers run on it
   void* pool_base = nullptr;
   Pool* pool = nullptr;
   std::vector<uint32_t*> d_twiddles;  // per log - ntt_min_log
@@ -182,7 +183,6 @@ struct Engine {
         err = ssip_ntt(p, d_twiddles[r.log_n - cfg.ntt_min_log], r.log_n, s, 8, 8);
       }
       if (err == hipSuccess) err = hipMemcpyAsync(r.output, x, n * 32, hipMemcpyDeviceToHost, s);
-      if (err == hipSuccess) err = hipStreamSynchronize(s);
     } else {
       uint32_t* sc = reinterpret_cast<uint32_t*>(base);
       char* ws = base + ((n * 32 + 255) & ~(size_t)255);
@@ -192,6 +192,10 @@ struct Engine {
       if (err == hipSuccess)
         err = zkp_msm_device(d_bases, sc, n, cfg.msm_window, target, ws, ws_bytes, s, r.output, 0);
     }
+    // Whatever failed, work already queued on s may still read or write the
+    // region: drain the stream before the region goes back to the pool.
+    const hipError_t sync = hipStreamSynchronize(s);
+    if (err == hipSuccess) err = sync;
     r.finish_s = now();
     release(off, bytes);
     return err;
@@ -214,7 +218,8 @@ hipError_t zkp_serve_create(const ZkpServeConfig* cfg, void** out) {
     return hipErrorInvalidValue;
   Engine* e = new Engine();
   e->cfg = *cfg;
-  hipError_t err = hipMalloc(&e->pool_base, cfg->hbm_budget);
+  hipError_t err = hipGetDevice(&e->device);
+  if (err == hipSuccess) err = hipMalloc(&e->pool_base, cfg->hbm_budget);
   if (err == hipSuccess) e->pool = new Pool(cfg->hbm_budget);
   for (unsigned k = cfg->ntt_min_log; k <= cfg->ntt_max_log && err == hipSuccess; k++) {
     uint32_t* t = nullptr;
@@ -272,7 +277,10 @@ hipError_t zkp_serve_run(void* engine, ZkpServeRequest* reqs, unsigned int count
   std::vector<std::thread> workers;
   for (unsigned w = 0; w < e->cfg.slots; w++) {
     workers.emplace_back([e, w, &first_error] {
-      (void)hipSetDevice(0);
+      if (hipSetDevice(e->device) != hipSuccess) {
+        int expected = 0;
+        first_error.compare_exchange_strong(expected, (int)hipErrorInvalidDevice);
+      }
       while (true) {
         ZkpServeRequest* r = nullptr;
         {

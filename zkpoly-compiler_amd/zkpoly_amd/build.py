@@ -161,15 +161,8 @@ def build_serve() -> str:
     return out
 
 
-def build_intpeak() -> str:
-    os.makedirs(os.path.join(PACKAGE_ROOT, "build"), exist_ok=True)
-    out = os.path.join(PACKAGE_ROOT, "build", "intpeak")
-    _run([hipcc(), *HIP_FLAGS, os.path.join(CSRC, "bench", "intpeak.hip"), "-o", out])
-    return out
-
-
 def targets() -> list:
-    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_fused_pipeline, build_intpeak]
+    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_fused_pipeline]
 
 
 def build_all(jobs: int | None = None) -> list[str]:
