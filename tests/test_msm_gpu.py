@@ -182,6 +182,49 @@ def test_workload_2_20(libs, stream, variant):
         assert list(run_device(libs, stream, pb, ss, window)) == list(want), window
 
 
+@pytest.mark.parametrize("window", [13, 14, 15, 17])
+def test_workload_2_20_other_windows(libs, stream, window):
+    """The rest of config 3's sweep (12, 16 and 18 run above) at 2^20 points."""
+    n = 1 << 20
+    ks = random_elements(n, 3000)
+    ss = random_elements(n, 3001)
+    pb = gpu_bases(libs, stream, ks)
+    assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss)), window
+
+
+@pytest.mark.parametrize("window,target,parts", [(16, 16, 1), (16, 16, 2), (13, 10, 3)])
+def test_reference_abi_2_20(libs, stream, window, target, parts):
+    """The drop-in msm symbol at 2^20: host points and two batches of host
+    scalars (the second with 10 % zeros and 10 % ones), the points streamed in
+    `parts` pieces; with target < windows the precomputed table of
+    msm_precompute is used too."""
+    n, batches = 1 << 20, 2
+    ks = random_elements(n, 9100)
+    pb = gpu_bases(libs, stream, ks)
+    t0 = np.ascontiguousarray(pb.download(np.uint64, n * 8).reshape(n, 8))
+    del pb
+    scal = [random_elements(n, 9200 + b) for b in range(batches)]
+    rng = np.random.Generator(np.random.PCG64(9300))
+    perm = rng.permutation(n)
+    scal[1][perm[: n // 10]] = 0
+    scal[1][perm[n // 10 : n // 5]] = mont_limbs(1)
+    cfg = MsmConfig(window_size=window, target_window=target, parts=parts)
+    n_pre = cfg.get_precompute()
+    tables = [t0] + [np.zeros_like(t0) for _ in range(n_pre - 1)]
+    arrays = [PointArray(t.ctypes.data, n) for t in tables]
+    if n_pre > 1:
+        MSMPrecompute(libs, cfg).get_fn()(arrays, n, 1)
+    op = MSM(libs, cfg)
+    size = op.get_buffer_size(n)[0]
+    buf = hip.DeviceBuffer(size)
+    results = [Point() for _ in range(batches)]
+    var = arrays + [ScalarArray(n, s.ctypes.data, "cpu") for s in scal]
+    print(f"msm 2^20 window={window} target={target} parts={parts}", flush=True)
+    op.get_fn().f([GpuBuffer(buf.addr, size)] + results, var, lambda c: c)
+    for b in range(batches):
+        assert list(results[b].data) == list(closed_form(ks, scal[b])), f"batch {b}"
+
+
 def test_workload_2_20_matches_best_multiexp(libs, stream):
     """One 2^20 instance also against the restated best_multiexp itself."""
     n = 1 << 20

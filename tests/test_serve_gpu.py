@@ -3,6 +3,8 @@ footprint.  Every result is bit-exact against the oracle and identical to the
 same request served alone; with a tight memory budget the requests that are
 in flight together never exceed it.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -25,7 +27,7 @@ def expected_for(res: Resident, trace: Trace):
         kind, log_n, _ = key
         x = trace.input(key)
         if kind == 0:
-            want[key] = orc.ntt(x, log_n)
+            want[key] = orc.ntt(x, log_n, min(16, os.cpu_count() or 1))
         else:
             n = 1 << log_n
             total = from_mont(orc.to_int(orc.inner_product(res.base_scalars[:n], x)))
@@ -63,6 +65,40 @@ def test_mix_equals_alone_and_oracle(libs, resident):
         trace.outputs[i].view()[:] = 0
         alone.run([r])
         assert np.array_equal(trace.output(i), before), i
+
+
+@pytest.fixture(scope="module")
+def resident_full(libs, stream):
+    """The serving workload's resident state: twiddles for NTT 2^14..2^22, one 2^21 base table."""
+    return make_resident(libs, stream, ntt_min=14, ntt_max=22, msm_max=21, seed=55)
+
+
+def test_workload_sizes_mix_equals_alone_and_oracle(libs, resident_full):
+    """Config 5 at its sizes: a Poisson mix of NTT 2^14..2^22 and MSM 2^16..2^21
+    (70 / 30) through four slots; every result equals the oracle, and each
+    request served alone afterwards gives the same bytes."""
+    trace = Trace(rate=1000.0, count=32, seed=1, variants=1, ntt_range=(14, 22), msm_range=(16, 21))
+    print("serve mix:", [(j.kind, j.log_n) for j in trace.jobs], flush=True)
+    want = expected_for(resident_full, trace)
+    srv = Server(libs, resident_full, slots=4, hbm_budget=16 << 30, msm_window=16)
+    reqs = trace.requests()
+    srv.run(reqs)
+    check_all(trace, reqs, want)
+    ev = sorted([(r.start_s, 1) for r in reqs] + [(r.finish_s, -1) for r in reqs])
+    live, peak = 0, 0
+    for _, d in ev:
+        live += d
+        peak = max(peak, live)
+    assert peak > 1
+    alone = Server(libs, resident_full, slots=1, hbm_budget=16 << 30, msm_window=16)
+    for i in range(len(trace.jobs)):
+        r = trace.requests()[i]
+        r.arrival_s = 0.0
+        trace.outputs[i].view()[:] = 0
+        alone.run([r])
+        assert r.status == 0
+        j = trace.jobs[i]
+        assert np.array_equal(trace.output(i), want[(j.kind, j.log_n, j.input_id)]), i
 
 
 def test_admission_respects_hbm_budget(libs, resident):
