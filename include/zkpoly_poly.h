@@ -59,6 +59,16 @@ hipError_t scan_mul(void* temp_buffer, unsigned long* buffer_size, PolyPtr targe
  * poly.h:22 (batched_invert.cuh:49-91). */
 hipError_t batched_invert(void* temp_buffer, unsigned long* buffer_size, PolyPtr poly, hipStream_t stream);
 
+/* batched_invert without a temporary buffer: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The temporaries live in a static
+ * device array of the library (about 3 MiB, one copy per GPU, held until the
+ * process ends), so poly.len is limited to 2^26 elements; longer views return
+ * hipErrorInvalidValue without a launch.  Nothing is allocated.  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream,
+ * or events).  Same results as batched_invert. */
+hipError_t zkp_batched_invert_noalloc(PolyPtr poly, hipStream_t stream);
+
 /* *target = *target^-1 on the device; zero stays zero.  Replaces inv_scalar,
  * poly.h:24. */
 hipError_t inv_scalar(unsigned int* target, hipStream_t stream);

@@ -35,6 +35,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "scan.hpp"
 
 namespace zkp {
@@ -332,6 +334,71 @@ static hipError_t fill(PolyPtr r, hipStream_t stream, int mode) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- batched inversion launch
+
+static size_t invert_part_bytes(uint64_t n) { return align256(tiles_of(n ? n : 1) * sizeof(Fr)); }
+
+// partial tile folds, pre, suf (one Fr per tile each), then the inverse of the total
+static size_t invert_temp_bytes(uint64_t n) { return 3 * invert_part_bytes(n) + align256(sizeof(Fr)); }
+
+static_assert(sizeof(ProdOp::T) == sizeof(Fr), "tile folds are stored as Fr");
+
+// The three launches of batched_invert over a temporary region t of at least
+// invert_temp_bytes(v.len).  Every byte the kernels read is written by an
+// earlier launch of the same call, so t needs no clearing.
+static hipError_t invert_launch(const PolyView& v, char* t, hipStream_t stream) {
+  const uint64_t n = v.len;
+  if (n == 0) return hipSuccess;
+  if (v.ptr == nullptr) return hipErrorInvalidValue;
+  const uint64_t nb = tiles_of(n);
+  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
+  const size_t part = invert_part_bytes(n);
+  auto* partial = reinterpret_cast<ProdOp::T*>(t);
+  Fr* pre = reinterpret_cast<Fr*>(t + part);
+  Fr* suf = reinterpret_cast<Fr*>(t + 2 * part);
+  Fr* inv = reinterpret_cast<Fr*>(t + 3 * part);
+  hipLaunchKernelGGL((k_tile_fold<ProdOp, SrcNonZero>), dim3((unsigned)nb), dim3(TH), 0, stream, SrcNonZero{v}, n,
+                     partial);
+  hipLaunchKernelGGL(k_invert_carries, dim3(1), dim3(TH), 0, stream, partial, nb, pre, suf, inv);
+  hipLaunchKernelGGL(k_invert_tile, dim3((unsigned)nb), dim3(TH), 0, stream, v, n, pre, suf, inv);
+  return hipGetLastError();
+}
+
+// Scratch of zkp_batched_invert_noalloc: room for the temporaries of the
+// largest accepted polynomial, ZKP_INVERT_NOALLOC_MAX_LEN elements (2^26:
+// 3 x 1 MiB of tile products + one element).  Every GPU the library runs on
+// holds its own copy until the process ends.  In host code the name `scratch`
+// is only a stand-in for the symbol; its device address per GPU comes from
+// hipGetSymbolAddress and is the only form ever passed to a kernel.
+#define ZKP_INVERT_NOALLOC_MAX_LEN (1ull << 26)
+#define ZKP_INVERT_SCRATCH_BYTES (3 * (ZKP_INVERT_NOALLOC_MAX_LEN / TILE) * 32 + 256)
+static __device__ __align__(256) unsigned char scratch[ZKP_INVERT_SCRATCH_BYTES];
+
+static_assert(ZKP_INVERT_NOALLOC_MAX_LEN % TILE == 0, "the maximum is a whole number of tiles");
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices];  // device address of `scratch` per GPU, once resolved
+
+// The current device's address of `scratch`, resolved on the first call on
+// that device (with that device current) and kept for the process lifetime.
+static hipError_t invert_scratch(char** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (ptr == nullptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = static_cast<char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 extern "C" {
 
 // r = a + b (poly.h:4)
@@ -436,27 +503,25 @@ hipError_t scan_mul(void* temp_buffer, unsigned long* buffer_size, PolyPtr targe
 hipError_t batched_invert(void* temp_buffer, unsigned long* buffer_size, PolyPtr poly, hipStream_t stream) {
   PolyView v;
   if (!make_view(poly, &v)) return hipErrorInvalidValue;
-  const uint64_t n = v.len;
-  const uint64_t nb = tiles_of(n ? n : 1);
-  const size_t part = align256(nb * sizeof(Fr));
   if (temp_buffer == nullptr) {
     if (buffer_size == nullptr) return hipErrorInvalidValue;
-    *buffer_size = 3 * part + align256(sizeof(Fr));
+    *buffer_size = invert_temp_bytes(v.len);
     return hipSuccess;
   }
-  if (n == 0) return hipSuccess;
-  if (v.ptr == nullptr) return hipErrorInvalidValue;
-  if (nb > 0x7fffffffull) return hipErrorInvalidValue;
-  char* t = reinterpret_cast<char*>(temp_buffer);
-  auto* partial = reinterpret_cast<ProdOp::T*>(t);
-  Fr* pre = reinterpret_cast<Fr*>(t + part);
-  Fr* suf = reinterpret_cast<Fr*>(t + 2 * part);
-  Fr* inv = reinterpret_cast<Fr*>(t + 3 * part);
-  hipLaunchKernelGGL((k_tile_fold<ProdOp, SrcNonZero>), dim3((unsigned)nb), dim3(TH), 0, stream, SrcNonZero{v}, n,
-                     partial);
-  hipLaunchKernelGGL(k_invert_carries, dim3(1), dim3(TH), 0, stream, partial, nb, pre, suf, inv);
-  hipLaunchKernelGGL(k_invert_tile, dim3((unsigned)nb), dim3(TH), 0, stream, v, n, pre, suf, inv);
-  return hipGetLastError();
+  return invert_launch(v, reinterpret_cast<char*>(temp_buffer), stream);
+}
+
+// batched_invert without a temporary buffer: the scratch is the library's
+// static device array (one per GPU), so nothing is allocated on any call.
+hipError_t zkp_batched_invert_noalloc(PolyPtr poly, hipStream_t stream) {
+  PolyView v;
+  if (!make_view(poly, &v)) return hipErrorInvalidValue;
+  if (v.len > ZKP_INVERT_NOALLOC_MAX_LEN || invert_temp_bytes(v.len) > ZKP_INVERT_SCRATCH_BYTES)
+    return hipErrorInvalidValue;
+  char* t = nullptr;
+  const hipError_t e = invert_scratch(&t);
+  if (e != hipSuccess) return e;
+  return invert_launch(v, t, stream);
 }
 
 // *target = *target^-1, zero stays zero (poly.h:24)
