@@ -54,6 +54,20 @@ hipError_t zkp_msm_device(const unsigned int* d_points, const unsigned int* d_sc
                           unsigned int window, unsigned int target_windows, void* d_workspace,
                           unsigned long long workspace_bytes, hipStream_t stream, unsigned int* h_result, int debug);
 
+/* zkp_msm_device without its host step, for batching MSMs into one stream or
+ * hipGraph: enqueues the device work on `stream` and, if h_sums is not NULL,
+ * an asynchronous copy of the n_windows window sums (128-byte XYZZ points)
+ * into pinned host memory.  Allocates nothing and never waits.  Run one MSM
+ * on the device before capturing this call into a graph (the first one sets
+ * the scatter kernels' LDS limits). */
+hipError_t zkp_msm_device_enqueue(const unsigned int* d_points, const unsigned int* d_scalars, unsigned long long len,
+                                  unsigned int window, unsigned int target_windows, void* d_workspace,
+                                  unsigned long long workspace_bytes, hipStream_t stream, void* h_sums);
+
+/* The affine result (h_result[16]) from window sums zkp_msm_device_enqueue copied out. */
+hipError_t zkp_msm_combine(unsigned int window, unsigned int target_windows, const void* h_sums,
+                           unsigned int* h_result);
+
 /* d_out[i] = k_i * G, G = (1, 2), for Fr Montgomery-form scalars (device, asynchronous). */
 hipError_t zkp_g1_mul_generator(const unsigned int* d_scalars, unsigned long long n, unsigned int* d_out,
                                 hipStream_t stream);

@@ -354,3 +354,15 @@ def test_poly_operators_refuse_small_temporary_buffers():
     ev = PolyEval(Libs())
     with pytest.raises(AssertionError, match="too small"):
         ev.get_fn().f([GpuBuffer(0x2000, 16), Scalar(0x3000)], [arr, Scalar(0x4000), None])
+
+
+def test_reference_abi_sizing_grows_with_stages_and_sets(libs):
+    """The sizing call counts stage_points point buffers, stage_scalars scalar
+    buffers and batch_per_run bucket sets."""
+    n = 1 << 16
+    base = MSM(libs, MsmConfig(window_size=16, target_window=16)).get_buffer_size(n)[0]
+    two_sets = MSM(libs, MsmConfig(window_size=16, target_window=16, batch_per_run=2)).get_buffer_size(n)[0]
+    staged = MSM(libs, MsmConfig(window_size=16, target_window=16, stage_points=2, stage_scalars=2)).get_buffer_size(n)[0]
+    bucket_set = 16 * (1 << 15) * 128
+    assert two_sets - base == bucket_set
+    assert staged - base == n * 64 + n * 32

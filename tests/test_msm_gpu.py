@@ -144,6 +144,42 @@ def test_reference_abi_parts_and_precompute(libs, stream):
         assert list(res.data) == list(want), (window, target, parts)
 
 
+@pytest.mark.parametrize("batches,bpr,parts,ss,sp", [(5, 2, 3, 2, 2), (4, 3, 1, 2, 1), (3, 1, 4, 1, 1), (6, 4, 2, 3, 3)])
+def test_reference_abi_runs_and_stages(libs, stream, batches, bpr, parts, ss, sp):
+    """batch_per_run batches share one pass over the parts (each run reverses
+    the previous run's order, so its first part is still resident), with
+    stage_scalars / stage_points rotating buffers on copy streams; host arrays
+    in pinned memory from the CPU pool.  Every batch against the closed form."""
+    from zkpoly_amd.pool import CpuMemoryPool
+
+    n = 10007
+    ks = random_elements(n, 8500 + batches)
+    pb = gpu_bases(libs, stream, ks)
+    pool = CpuMemoryPool(20, 64, libs)  # 64 MiB pinned slabs
+    pts_ptr = pool.allocate(n * 64)
+    pool.view(pts_ptr, n * 64)[:] = pb.download(np.uint64, n * 8)
+    scal, arrays = [], []
+    for b in range(batches):
+        s_host = random_elements(n, 8600 + 10 * batches + b)
+        if b == 1:
+            s_host[::9] = 0
+        ptr = pool.allocate(n * 32)
+        pool.view(ptr, n * 32)[:] = s_host.reshape(-1)
+        scal.append(s_host)
+        arrays.append(ScalarArray(n, ptr, "cpu"))
+    cfg = MsmConfig(window_size=13, target_window=20, batch_per_run=bpr, parts=parts, stage_scalars=ss,
+                    stage_points=sp)
+    op = MSM(libs, cfg)
+    size = op.get_buffer_size(n)[0]
+    buf = hip.DeviceBuffer(size)
+    results = [Point() for _ in range(batches)]
+    print(f"msm runs batches={batches} bpr={bpr} parts={parts} stages={ss}/{sp}", flush=True)
+    op.get_fn().f([GpuBuffer(buf.addr, size)] + results, [PointArray(pts_ptr, n)] + arrays, lambda c: c)
+    for b in range(batches):
+        assert list(results[b].data) == list(closed_form(ks, scal[b])), f"batch {b}"
+    pool.destroy()
+
+
 def test_parts_with_a_heavy_bucket(libs, stream):
     """Window 16 in two parts with 30 % of the scalars equal to 1: the first
     part stores its buckets and the second adds to them, while the scalar-1

@@ -35,6 +35,8 @@
 #include <string.h>
 
 #include <chrono>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../curve/g1.hpp"
@@ -674,6 +676,21 @@ __global__ void __launch_bounds__(THREADS) k_fill_identity(u32* pts, uint64_t co
     reinterpret_cast<uint4*>(pts)[i] = z;
 }
 
+// dst[i] += src[i] over H buckets (XYZZ): a later part's buckets, accumulated
+// into a scratch set of their own, join the batch's buckets in one pass with
+// no divergence, instead of every run of that part reading and adding its
+// bucket inside the accumulation loop.
+__global__ void __launch_bounds__(THREADS) k_merge_buckets(u32* __restrict__ dst, const u32* __restrict__ src,
+                                                           uint32_t H) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= H) return;
+  const G1XYZZ b = G1XYZZ::load(src + (size_t)i * 32);
+  if (b.is_identity()) return;
+  u32* d = dst + (size_t)i * 32;
+  const G1XYZZ a = G1XYZZ::load(d);
+  (a.is_identity() ? b : a.add(b)).store(d);
+}
+
 __global__ void k_set_count(uint32_t* counts, const uint32_t* offsets, uint32_t h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counts[0] = offsets[h];
 }
@@ -794,6 +811,8 @@ struct Layout {
   int levels;  // accumulation levels after level 0
   uint32_t red_m[MAX_LEVELS];
   int red_levels;  // one-level launches before the in-workgroup tail
+  size_t bucket_bytes;  // one bucket set (n_windows * 2^(s-1) XYZZ points)
+  uint32_t sets;        // bucket sets from `buckets` on (one per batch of a run)
 };
 
 // Partition-sort parameters for `len` points; false when the window is too wide
@@ -819,8 +838,10 @@ static bool make_sort_plan(const Config& cfg, uint64_t len, SortPlan* sp) {
   return sp->tiles > 0;
 }
 
-static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
+static bool make_layout(const Config& cfg, uint64_t len, Layout* L, uint32_t sets = 1) {
   memset(L, 0, sizeof(*L));
+  if (sets == 0) return false;
+  L->sets = sets;
   L->H = cfg.n_windows * cfg.buckets;
   L->nb_scan = div_ceil32((uint64_t)L->H + 1, SCAN_BLOCK);
   if (L->nb_scan > THREADS * 1024) return false;
@@ -881,7 +902,8 @@ static bool make_layout(const Config& cfg, uint64_t len, Layout* L) {
   L->lvl_pts[0] = off; off = align256(off + max_even * 128);
   L->lvl_keys[1] = off; off = align256(off + max_odd * 4);
   L->lvl_pts[1] = off; off = align256(off + max_odd * 128);
-  L->buckets = off; off = align256(off + (size_t)L->H * 128);
+  L->bucket_bytes = align256((size_t)L->H * 128);
+  L->buckets = off; off += L->bucket_bytes * sets;
   // reduction ping-pong: X and Y for even / odd levels
   L->red[0] = off; off = align256(off + (size_t)cfg.n_windows * red_max0 * 128);
   L->red[1] = off; off = align256(off + (size_t)cfg.n_windows * red_max0 * 128);
@@ -896,6 +918,26 @@ static inline unsigned grid_for(uint64_t work, unsigned cap) {
   uint64_t g = (work + THREADS - 1) / THREADS;
   if (g > cap) g = cap;
   return g == 0 ? 1u : (unsigned)g;
+}
+
+// The partition and segment scatters take more than 64 KB of dynamic LDS; the
+// permission is set once per device, at each kernel's largest request, on the
+// first MSM there (outside any graph capture a caller may start later).
+static std::mutex g_attr_mu;
+static bool g_attr_done[64];
+
+static hipError_t ensure_lds_limits() {
+  int dev = 0;
+  ZKP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_attr_mu);
+  if (g_attr_done[dev]) return hipSuccess;
+  ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)PART_LDS_MAX));
+  ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_seg_scatter),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)seg_scatter_lds(12)));
+  g_attr_done[dev] = true;
+  return hipSuccess;
 }
 
 struct PhaseTimer {
@@ -925,24 +967,30 @@ struct PhaseTimer {
   }
 };
 
-// Zeroes the buckets (fill kernel, safe under graph capture).
-static hipError_t reset_buckets(const Config& cfg, const Layout& L, char* ws, hipStream_t stream) {
+static inline u32* bucket_set(const Layout& L, char* ws, uint32_t set) {
+  return reinterpret_cast<u32*>(ws + L.buckets + (size_t)set * L.bucket_bytes);
+}
+
+// Zeroes bucket set `set` (fill kernel, safe under graph capture).
+static hipError_t reset_buckets(const Config& cfg, const Layout& L, char* ws, uint32_t set, hipStream_t stream) {
   hipLaunchKernelGGL(k_fill_identity, dim3(grid_for((uint64_t)L.H * 8, 16384)), dim3(THREADS), 0, stream,
-                     reinterpret_cast<u32*>(ws + L.buckets), (uint64_t)L.H);
+                     bucket_set(L, ws, set), (uint64_t)L.H);
   return hipGetLastError();
 }
 
-// Adds sum_i k_i P_i for one part (len points, n_precompute tables laid out
-// table-major at d_points) into the buckets.
-static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, const u32* d_points,
-                                  const u32* d_scalars, uint64_t len, bool fresh, hipStream_t stream, PhaseTimer& pt) {
+// First half of a part: the signed digits of its scalars sorted by bucket into
+// keys0 / vals0 (point index and sign), with the entry counts in `counts`.
+// Reads only the scalars, so it can run while the part's points are still
+// crossing the link.
+static hipError_t sort_part(const Config& cfg, const Layout& L, char* ws, const u32* d_scalars, uint64_t len,
+                            hipStream_t stream, PhaseTimer& pt) {
+  ZKP_CHECK(ensure_lds_limits());
   uint32_t* hist = reinterpret_cast<uint32_t*>(ws + L.hist);
   uint32_t* cursor = reinterpret_cast<uint32_t*>(ws + L.cursor);
   uint32_t* bsums = reinterpret_cast<uint32_t*>(ws + L.block_sums);
   uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
   uint32_t* keys0 = reinterpret_cast<uint32_t*>(ws + L.keys0);
   uint32_t* vals0 = reinterpret_cast<uint32_t*>(ws + L.vals0);
-  u32* buckets = reinterpret_cast<u32*>(ws + L.buckets);
   if (L.part) {
     // partition sort (no device atomics); `len` may be below the planned maximum
     SortPlan sp = L.sp;
@@ -964,9 +1012,6 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, stream, counts, matrix, (uint32_t)(mn - 1));
     ZKP_CHECK(hipGetLastError());
     pt.mark("scan");
-    if (sp.lds_part > 64 * 1024)
-      ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sp.lds_part));
     hipLaunchKernelGGL(k_part_scatter, dim3(sp.tiles), dim3(THREADS), sp.lds_part, stream, d_scalars, len, cfg, sp,
                        matrix, ktmp, vtmp);
     ZKP_CHECK(hipGetLastError());
@@ -978,9 +1023,6 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
     hipLaunchKernelGGL(k_seg_hist, dim3(L.max_slices), dim3(THREADS), nbk * 4, stream, ktmp, matrix, slices, sp, cursor);
     hipLaunchKernelGGL(k_seg_scan, dim3(sp.nc), dim3(THREADS), nbk * 4, stream, matrix, sp, cursor);
     const uint32_t lds_seg = seg_scatter_lds(sp.low_bits);
-    if (lds_seg > 64 * 1024)
-      ZKP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_seg_scatter),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_seg));
     hipLaunchKernelGGL(k_seg_scatter, dim3(L.max_slices), dim3(THREADS), lds_seg, stream, ktmp, vtmp, matrix, slices,
                        sp, cursor, keys0, vals0);
     ZKP_CHECK(hipGetLastError());
@@ -1004,6 +1046,17 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
   ZKP_CHECK(hipGetLastError());
   pt.mark("scatter");
   }
+  return hipSuccess;
+}
+
+// Second half: adds the sorted entries' points (n_precompute tables laid out
+// table-major at d_points) into bucket set `set`.
+static hipError_t accumulate_sorted(const Config& cfg, const Layout& L, char* ws, uint32_t set, const u32* d_points,
+                                    bool fresh, hipStream_t stream, PhaseTimer& pt) {
+  uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+  uint32_t* keys0 = reinterpret_cast<uint32_t*>(ws + L.keys0);
+  uint32_t* vals0 = reinterpret_cast<uint32_t*>(ws + L.vals0);
+  u32* buckets = bucket_set(L, ws, set);
   const uint64_t nchunks0 = (L.n_entries_max + CHUNK - 1) / CHUNK;
   // with a single chunk every run is closed: no next level
   uint32_t* nk = L.levels > 0 ? reinterpret_cast<uint32_t*>(ws + L.lvl_keys[0]) : nullptr;
@@ -1042,10 +1095,18 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
   return hipSuccess;
 }
 
-// Bucket reduction; the n_windows window sums end at *out_dev (XYZZ).
-static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, hipStream_t stream, const u32** out_dev,
-                                 PhaseTimer& pt) {
-  const u32* X = reinterpret_cast<const u32*>(ws + L.buckets);
+// Adds sum_i k_i P_i for one part (len points, n_precompute tables laid out
+// table-major at d_points) into bucket set `set`.
+static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, uint32_t set, const u32* d_points,
+                                  const u32* d_scalars, uint64_t len, bool fresh, hipStream_t stream, PhaseTimer& pt) {
+  ZKP_CHECK(sort_part(cfg, L, ws, d_scalars, len, stream, pt));
+  return accumulate_sorted(cfg, L, ws, set, d_points, fresh, stream, pt);
+}
+
+// Reduction of bucket set `set`; the n_windows window sums end at *out_dev (XYZZ).
+static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, uint32_t set, hipStream_t stream,
+                                 const u32** out_dev, PhaseTimer& pt) {
+  const u32* X = bucket_set(L, ws, set);
   const u32* Y = X;
   uint32_t M = cfg.buckets;
   if (M < 2 || (M & (M - 1))) return hipErrorInvalidValue;  // 2^(s-1) buckets, s >= 2
@@ -1090,10 +1151,10 @@ static hipError_t fetch_window_sums(const Config& cfg, const u32* d_sums, hipStr
 static hipError_t msm_device(const Config& cfg, const u32* d_points, const u32* d_scalars, uint64_t len, char* ws,
                              const Layout& L, hipStream_t stream, G1XYZZ* result, bool debug) {
   PhaseTimer pt{debug, stream, {}, {}};
-  ZKP_CHECK(reset_buckets(cfg, L, ws, stream));
-  if (len > 0) ZKP_CHECK(accumulate_part(cfg, L, ws, d_points, d_scalars, len, true, stream, pt));
+  ZKP_CHECK(reset_buckets(cfg, L, ws, 0, stream));
+  if (len > 0) ZKP_CHECK(accumulate_part(cfg, L, ws, 0, d_points, d_scalars, len, true, stream, pt));
   const u32* d_sums = nullptr;
-  ZKP_CHECK(reduce_buckets(cfg, L, ws, stream, &d_sums, pt));
+  ZKP_CHECK(reduce_buckets(cfg, L, ws, 0, stream, &d_sums, pt));
   std::vector<G1XYZZ> sums;
   ZKP_CHECK(fetch_window_sums(cfg, d_sums, stream, sums));
   auto t0 = std::chrono::steady_clock::now();
@@ -1102,6 +1163,231 @@ static hipError_t msm_device(const Config& cfg, const u32* d_points, const u32* 
     pt.report();
     auto t1 = std::chrono::steady_clock::now();
     fprintf(stderr, "[msm] host combine %.4f ms\n", std::chrono::duration<double, std::milli>(t1 - t0).count());
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------- host-streamed MSM (the drop-in msm)
+//
+// Plan of one call: each card takes a contiguous range of points, split into
+// `parts` pieces that are streamed through its buffer; batches run in groups
+// of batch_per_run that share one pass over the parts, each with its own
+// bucket set.  Copies go on two copy streams (points, scalars) into
+// stage_points / stage_scalars rotating device buffers, ordered against the
+// compute stream by events, so the next part's points and the next batch's
+// scalars cross the link while the current accumulation runs.  Runs
+// alternate the direction of their pass over the parts, so the part a run
+// ends with is still resident when the next run starts with it (with one part
+// the points cross the link once per call).
+struct HostPlan {
+  uint64_t part_len;      // points per part (the last may be shorter)
+  uint32_t scratch_set;   // bucket set for the parts after a run's first (UINT32_MAX: one part)
+  uint32_t stage_points;  // rotating device buffers for point parts
+  uint32_t stage_scalars; // rotating device buffers for scalar parts
+  uint32_t batch_per_run; // bucket sets
+  size_t pts_bytes;       // one point stage: part_len * n_precompute affine points
+  size_t sc_bytes;        // one scalar stage: part_len Fr elements
+  Layout L;
+
+  size_t total() const { return stage_points * pts_bytes + stage_scalars * sc_bytes + L.total; }
+};
+
+static bool make_host_plan(const Config& cfg, uint64_t card_len, uint32_t parts, uint32_t batch_per_run,
+                           uint32_t stage_scalars, uint32_t stage_points, HostPlan* P) {
+  if (parts == 0 || batch_per_run == 0 || stage_scalars == 0 || stage_points == 0) return false;
+  if (batch_per_run > 1024 || stage_scalars > 64 || stage_points > 64) return false;
+  P->part_len = (card_len + parts - 1) / parts;
+  P->stage_points = stage_points;
+  P->stage_scalars = stage_scalars;
+  P->batch_per_run = batch_per_run;
+  const bool several = P->part_len < card_len;
+  P->scratch_set = several ? batch_per_run : UINT32_MAX;
+  if (!make_layout(cfg, P->part_len ? P->part_len : 1, &P->L, batch_per_run + (several ? 1 : 0))) return false;
+  P->pts_bytes = align256((size_t)P->part_len * cfg.n_precompute * 64);
+  P->sc_bytes = align256((size_t)P->part_len * 32);
+  return true;
+}
+
+// One card's streams (compute, point copies, scalar copies), the events that
+// order them, and pinned slots for the window sums.  Made once per device and
+// kept for the process's lifetime in a free list (stream, event and pinned
+// allocations cost more than a small MSM); a call takes one set for itself,
+// grows it if its plan needs more stages, and gives it back drained.
+struct CardRes {
+  hipStream_t s = nullptr, cp = nullptr, cs = nullptr;
+  std::vector<hipEvent_t> pt_ready, pt_free, sc_ready, sc_free;
+  G1XYZZ* h_sums = nullptr;  // batch_per_run x n_windows window sums (pinned)
+  size_t sums_cap = 0;
+
+  hipError_t ensure(const Config& cfg, const HostPlan& P) {
+    if (s == nullptr) ZKP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (cp == nullptr) ZKP_CHECK(hipStreamCreateWithFlags(&cp, hipStreamNonBlocking));
+    if (cs == nullptr) ZKP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    auto grow = [](std::vector<hipEvent_t>& v, uint32_t n) -> hipError_t {
+      while (v.size() < n) {
+        hipEvent_t e;
+        ZKP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        v.push_back(e);
+      }
+      return hipSuccess;
+    };
+    ZKP_CHECK(grow(pt_ready, P.stage_points));
+    ZKP_CHECK(grow(pt_free, P.stage_points));
+    ZKP_CHECK(grow(sc_ready, P.stage_scalars));
+    ZKP_CHECK(grow(sc_free, P.stage_scalars));
+    const size_t need = sizeof(G1XYZZ) * cfg.n_windows * P.batch_per_run;
+    if (sums_cap < need) {
+      if (h_sums) ZKP_CHECK(hipHostFree(h_sums));
+      h_sums = nullptr;
+      sums_cap = 0;
+      ZKP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_sums), need, hipHostMallocDefault));
+      sums_cap = need;
+    }
+    // every stage starts free
+    for (uint32_t i = 0; i < P.stage_points; i++) ZKP_CHECK(hipEventRecord(pt_free[i], s));
+    for (uint32_t i = 0; i < P.stage_scalars; i++) ZKP_CHECK(hipEventRecord(sc_free[i], s));
+    return hipSuccess;
+  }
+  void drain() {
+    for (hipStream_t q : {s, cp, cs})
+      if (q) (void)hipStreamSynchronize(q);
+  }
+};
+
+static std::mutex g_res_mu;
+static std::vector<CardRes*>* g_res_free = new std::vector<CardRes*>[64];  // per device; never freed
+
+struct ResLease {
+  int dev;
+  CardRes* r = nullptr;
+  explicit ResLease(int d) : dev(d) {
+    std::lock_guard<std::mutex> lock(g_res_mu);
+    auto& v = g_res_free[dev];
+    if (!v.empty()) {
+      r = v.back();
+      v.pop_back();
+    }
+    if (r == nullptr) r = new CardRes;
+  }
+  ~ResLease() {
+    r->drain();  // nothing of this call may still run when the set is reused
+    std::lock_guard<std::mutex> lock(g_res_mu);
+    g_res_free[dev].push_back(r);
+  }
+};
+
+// One card's share: points [c_off, c_off + c_len) of every table, all
+// batches; adds the card's partial result of batch b into totals[b].
+//
+// Per part, each batch's scalars are copied first; the part's sort reads only
+// them, so it runs while the part's points are still on the link; the
+// accumulation waits for the points.  The next part's points are requested
+// as soon as this part's first scalars are on their way, into another point
+// stage, so they cross the link during this part's sorts and accumulations.
+static hipError_t msm_card(const Config& cfg, const HostPlan& P, unsigned card, char* buf, uint64_t c_off,
+                           uint64_t c_len, const u32* const* h_points, unsigned batches, const u32* const* h_scalars,
+                           G1XYZZ* totals, bool debug) {
+  ZKP_CHECK(hipSetDevice((int)card));
+  int dev = 0;
+  ZKP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  ResLease lease(dev);
+  CardRes& R = *lease.r;
+  ZKP_CHECK(R.ensure(cfg, P));
+  const Layout& L = P.L;
+  char* ws = buf + P.stage_points * P.pts_bytes + P.stage_scalars * P.sc_bytes;
+  auto pts = [&](uint32_t st) { return reinterpret_cast<u32*>(buf + st * P.pts_bytes); };
+  auto scs = [&](uint32_t st) { return reinterpret_cast<u32*>(buf + P.stage_points * P.pts_bytes + st * P.sc_bytes); };
+  const uint32_t nparts = c_len ? (uint32_t)((c_len + P.part_len - 1) / P.part_len) : 0;
+  auto part_off = [&](uint32_t p) { return c_off + (uint64_t)p * P.part_len; };
+  auto part_len = [&](uint32_t p) {
+    const uint64_t o = part_off(p), end = c_off + c_len;
+    return end - o < P.part_len ? end - o : P.part_len;
+  };
+  // which part each point stage holds (-1: none); the copy has been enqueued
+  std::vector<int64_t> held(P.stage_points, -1);
+  std::vector<uint64_t> last_use(P.stage_points, 0);
+  uint64_t clock = 0;
+  uint32_t sc_next = 0;
+  // stage holding part p, enqueueing its copy into the least recently used
+  // stage other than `keep` when it is not resident; `after` (may be null) is
+  // an event the copy waits for first (the scalars that should go first)
+  auto points_of = [&](uint32_t p, int64_t keep, hipEvent_t after, uint32_t* out) -> hipError_t {
+    for (uint32_t st = 0; st < P.stage_points; st++)
+      if (held[st] == (int64_t)p) {
+        *out = st;
+        last_use[st] = ++clock;
+        return hipSuccess;
+      }
+    uint32_t st = P.stage_points;
+    for (uint32_t k = 0; k < P.stage_points; k++)
+      if ((int64_t)k != keep && (st == P.stage_points || last_use[k] < last_use[st])) st = k;
+    if (st == P.stage_points) st = 0;  // one stage: reuse it after its last reader
+    ZKP_CHECK(hipStreamWaitEvent(R.cp, R.pt_free[st], 0));
+    if (after != nullptr) ZKP_CHECK(hipStreamWaitEvent(R.cp, after, 0));
+    const uint64_t off = part_off(p), n = part_len(p);
+    for (uint32_t t = 0; t < cfg.n_precompute; t++)
+      ZKP_CHECK(hipMemcpyAsync(pts(st) + (size_t)t * n * 16, h_points[t] + off * 16, n * 64, hipMemcpyHostToDevice,
+                               R.cp));
+    ZKP_CHECK(hipEventRecord(R.pt_ready[st], R.cp));
+    held[st] = p;
+    last_use[st] = ++clock;
+    *out = st;
+    return hipSuccess;
+  };
+  for (unsigned r0 = 0, run = 0; r0 < batches; r0 += P.batch_per_run, run++) {
+    const uint32_t nb = batches - r0 < P.batch_per_run ? batches - r0 : P.batch_per_run;
+    PhaseTimer pt{debug, R.s, {}, {}};
+    for (uint32_t j = 0; j < nb; j++) ZKP_CHECK(reset_buckets(cfg, L, ws, j, R.s));
+    const bool forward = (run & 1) == 0;
+    for (uint32_t idx = 0; idx < nparts; idx++) {
+      const uint32_t p = forward ? idx : nparts - 1 - idx;
+      const uint64_t off = part_off(p), n = part_len(p);
+      int64_t st = -1;
+      for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t ss = sc_next++ % P.stage_scalars;
+        ZKP_CHECK(hipStreamWaitEvent(R.cs, R.sc_free[ss], 0));
+        ZKP_CHECK(hipMemcpyAsync(scs(ss), h_scalars[r0 + j] + off * 8, n * 32, hipMemcpyHostToDevice, R.cs));
+        ZKP_CHECK(hipEventRecord(R.sc_ready[ss], R.cs));
+        if (j == 0) {
+          // the part's points after its first scalars (resident parts are not copied again), then the next
+          // part's points into another stage
+          uint32_t s0 = 0;
+          ZKP_CHECK(points_of(p, -1, R.sc_ready[ss], &s0));
+          st = s0;
+          if (idx + 1 < nparts && P.stage_points > 1) {
+            uint32_t s1 = 0;
+            ZKP_CHECK(points_of(forward ? p + 1 : p - 1, st, nullptr, &s1));
+          }
+        }
+        ZKP_CHECK(hipStreamWaitEvent(R.s, R.sc_ready[ss], 0));
+        ZKP_CHECK(sort_part(cfg, L, ws, scs(ss), n, R.s, pt));
+        ZKP_CHECK(hipEventRecord(R.sc_free[ss], R.s));
+        if (j == 0) ZKP_CHECK(hipStreamWaitEvent(R.s, R.pt_ready[st], 0));
+        if (idx == 0 || P.scratch_set == UINT32_MAX) {
+          ZKP_CHECK(accumulate_sorted(cfg, L, ws, j, pts((uint32_t)st), idx == 0, R.s, pt));
+        } else {
+          ZKP_CHECK(reset_buckets(cfg, L, ws, P.scratch_set, R.s));
+          ZKP_CHECK(accumulate_sorted(cfg, L, ws, P.scratch_set, pts((uint32_t)st), true, R.s, pt));
+          hipLaunchKernelGGL(k_merge_buckets, dim3(div_ceil32(L.H, THREADS)), dim3(THREADS), 0, R.s,
+                             bucket_set(L, ws, j), bucket_set(L, ws, P.scratch_set), L.H);
+          ZKP_CHECK(hipGetLastError());
+        }
+      }
+      ZKP_CHECK(hipEventRecord(R.pt_free[st], R.s));
+    }
+    for (uint32_t j = 0; j < nb; j++) {
+      const u32* d_sums = nullptr;
+      ZKP_CHECK(reduce_buckets(cfg, L, ws, j, R.s, &d_sums, pt));
+      ZKP_CHECK(hipMemcpyAsync(R.h_sums + (size_t)j * cfg.n_windows, d_sums, sizeof(G1XYZZ) * cfg.n_windows,
+                               hipMemcpyDeviceToHost, R.s));
+    }
+    ZKP_CHECK(hipStreamSynchronize(R.s));
+    pt.report();
+    for (uint32_t j = 0; j < nb; j++) {
+      std::vector<G1XYZZ> sums(R.h_sums + (size_t)j * cfg.n_windows, R.h_sums + (size_t)(j + 1) * cfg.n_windows);
+      totals[r0 + j] = totals[r0 + j].add(combine_windows(cfg, sums));
+    }
   }
   return hipSuccess;
 }
@@ -1116,81 +1402,59 @@ extern "C" {
 
 // Multi-scalar multiplication of host points / host scalars, results to host
 // (affine), for MsmConfig (window, target_windows).  Behind msm of
-// core/src/msm/src/msm.h:4-9 / msm.cu:9-41, with the same calling convention: with buffers == NULL only buffer_sizes[card] (bytes
-// of device workspace per card) is filled.  Points are split over the cards in
-// contiguous parts, each card's part in `parts` pieces streamed through its
-// buffer; per-card results are summed on the host.  Synchronous.
-// h_points holds n_precompute tables (msm_precompute), h_scaler_batch `batches`
-// scalar arrays (Fr Montgomery, no rotation), h_result `batches` affine points.
+// core/src/msm/src/msm.h:4-9 / msm.cu:9-41, with the same calling convention:
+// with buffers == NULL only buffer_sizes[card] (bytes of device memory per
+// card) is filled.  Points are split over the cards in contiguous ranges, one
+// host thread per card (MultiGPUMSM, msm_impl.cuh:830-909); each card streams
+// its range in `parts` pieces with batch_per_run batches per pass, through
+// stage_points / stage_scalers rotating buffers on copy streams (MSM::run,
+// msm_impl.cuh:374-603); the cards' results are summed on the host.
+// Synchronous.  h_points holds n_precompute tables (msm_precompute),
+// h_scaler_batch `batches` scalar arrays (Fr Montgomery, no rotation),
+// h_result `batches` affine points.  Host arrays in pinned memory (as the
+// runtime's CPU pool gives) let the copies overlap the accumulation.
 hipError_t zkp_msm_host(unsigned int window, unsigned int target_windows, int debug_flag, void* const* buffers,
                         unsigned long* buffer_sizes, unsigned long long len, unsigned int batch_per_run,
                         unsigned int parts, unsigned int stage_scalers, unsigned int stage_points,
                         unsigned int num_cards, const unsigned int* cards, unsigned int const* const* h_points,
                         unsigned int batches, unsigned int const* const* h_scaler_batch,
                         unsigned int* const* h_result) {
-  (void)batch_per_run;
-  (void)stage_scalers;
-  (void)stage_points;
   Config cfg;
   if (!make_config(window, target_windows, &cfg)) return hipErrorInvalidValue;
-  if (num_cards == 0 || parts == 0) return hipErrorInvalidValue;
+  if (num_cards == 0 || cards == nullptr) return hipErrorInvalidValue;
   const uint64_t card_len = (len + num_cards - 1) / num_cards;
-  const uint64_t part_len = (card_len + parts - 1) / parts;
-  Layout L;
-  if (!make_layout(cfg, part_len ? part_len : 1, &L)) return hipErrorInvalidValue;
-  const size_t pts_bytes = align256((size_t)part_len * cfg.n_precompute * 64);
-  const size_t sc_bytes = align256((size_t)part_len * 32);
-  const size_t need = pts_bytes + sc_bytes + L.total;
+  HostPlan P;
+  if (!make_host_plan(cfg, card_len, parts, batch_per_run, stage_scalers, stage_points, &P))
+    return hipErrorInvalidValue;
   if (buffers == nullptr) {
     if (buffer_sizes == nullptr) return hipErrorInvalidValue;
-    for (unsigned c = 0; c < num_cards; c++) buffer_sizes[c] = need;
+    for (unsigned c = 0; c < num_cards; c++) buffer_sizes[c] = P.total();
     return hipSuccess;
   }
   if (batches > 0 && (h_points == nullptr || h_scaler_batch == nullptr || h_result == nullptr))
     return hipErrorInvalidValue;
-  std::vector<G1XYZZ> totals(batches, G1XYZZ::identity());
   const bool debug = debug_flag != 0;
+  std::vector<std::vector<G1XYZZ>> partial(num_cards, std::vector<G1XYZZ>(batches, G1XYZZ::identity()));
+  std::vector<hipError_t> errs(num_cards, hipSuccess);
+  std::vector<std::thread> threads;
   for (unsigned c = 0; c < num_cards; c++) {
     const uint64_t c_off = c * card_len;
     if (c_off >= len) break;
     const uint64_t c_len = len - c_off < card_len ? len - c_off : card_len;
-    ZKP_CHECK(hipSetDevice((int)cards[c]));
-    hipStream_t stream;
-    ZKP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    char* buf = reinterpret_cast<char*>(buffers[c]);
-    u32* d_pts = reinterpret_cast<u32*>(buf);
-    u32* d_sc = reinterpret_cast<u32*>(buf + pts_bytes);
-    char* ws = buf + pts_bytes + sc_bytes;
-    hipError_t err = hipSuccess;
-    const uint32_t nparts = (uint32_t)((c_len + part_len - 1) / part_len);
-    for (unsigned b = 0; b < batches && err == hipSuccess; b++) {
-      PhaseTimer pt{debug, stream, {}, {}};
-      err = reset_buckets(cfg, L, ws, stream);
-      for (uint32_t p = 0; p < nparts && err == hipSuccess; p++) {
-        const uint64_t p_off = c_off + (uint64_t)p * part_len;
-        const uint64_t p_len = c_off + c_len - p_off < part_len ? c_off + c_len - p_off : part_len;
-        // points of this part for every table, table-major with stride p_len
-        if (b == 0 || nparts > 1) {
-          for (uint32_t t = 0; t < cfg.n_precompute && err == hipSuccess; t++)
-            err = hipMemcpyAsync(d_pts + (size_t)t * p_len * 16, h_points[t] + p_off * 16, p_len * 64,
-                                 hipMemcpyHostToDevice, stream);
-        }
-        if (err == hipSuccess)
-          err = hipMemcpyAsync(d_sc, h_scaler_batch[b] + p_off * 8, p_len * 32, hipMemcpyHostToDevice, stream);
-        if (err == hipSuccess) err = accumulate_part(cfg, L, ws, d_pts, d_sc, p_len, p == 0, stream, pt);
-      }
-      const u32* d_sums = nullptr;
-      if (err == hipSuccess) err = reduce_buckets(cfg, L, ws, stream, &d_sums, pt);
-      std::vector<G1XYZZ> sums;
-      if (err == hipSuccess) err = fetch_window_sums(cfg, d_sums, stream, sums);
-      if (err == hipSuccess) totals[b] = totals[b].add(combine_windows(cfg, sums));
-      pt.report();
-    }
-    (void)hipStreamDestroy(stream);
-    if (err != hipSuccess) return err;
+    auto job = [&, c, c_off, c_len]() {
+      errs[c] = msm_card(cfg, P, cards[c], reinterpret_cast<char*>(buffers[c]), c_off, c_len, h_points, batches,
+                         h_scaler_batch, partial[c].data(), debug);
+    };
+    if (num_cards == 1) job();
+    else threads.emplace_back(job);
   }
+  for (auto& t : threads) t.join();
+  for (unsigned c = 0; c < num_cards; c++)
+    if (errs[c] != hipSuccess) return errs[c];
   for (unsigned b = 0; b < batches; b++) {
-    const G1Affine a = totals[b].to_affine();
+    G1XYZZ total = G1XYZZ::identity();
+    for (unsigned c = 0; c < num_cards; c++) total = total.add(partial[c][b]);
+    const G1Affine a = total.to_affine();
     memcpy(h_result[b], a.x.v, 32);
     memcpy(h_result[b] + 8, a.y.v, 32);
   }
@@ -1267,6 +1531,49 @@ hipError_t zkp_msm_device(const unsigned int* d_points, const unsigned int* d_sc
   ZKP_CHECK(msm_device(cfg, d_points, d_scalars, len, reinterpret_cast<char*>(d_workspace), L, stream, &r,
                        debug != 0));
   const G1Affine a = r.to_affine();
+  memcpy(h_result, a.x.v, 32);
+  memcpy(h_result + 8, a.y.v, 32);
+  return hipSuccess;
+}
+
+// zkp_msm_device split at the host boundary, for callers that batch MSMs into
+// one stream or hipGraph: enqueues the whole device part (digits, sort,
+// accumulation, bucket reduction) on `stream` and, when h_sums is not NULL,
+// an asynchronous copy of the n_windows window sums (XYZZ, 128 bytes each)
+// into h_sums (pinned host memory, n_windows * 128 bytes).  Nothing is
+// allocated and nothing waits; zkp_msm_combine turns the sums into the
+// result once the stream has passed the copy.  Launch permissions are set on
+// the device's first MSM: run one MSM (or this call) outside a capture first.
+hipError_t zkp_msm_device_enqueue(const unsigned int* d_points, const unsigned int* d_scalars, unsigned long long len,
+                                  unsigned int window, unsigned int target_windows, void* d_workspace,
+                                  unsigned long long workspace_bytes, hipStream_t stream, void* h_sums) {
+  Config cfg;
+  if (!make_config(window, target_windows, &cfg)) return hipErrorInvalidValue;
+  Layout L;
+  if (!make_layout(cfg, len ? len : 1, &L)) return hipErrorInvalidValue;
+  if (d_workspace == nullptr || workspace_bytes < L.total) return hipErrorInvalidValue;
+  if (len > 0 && (d_points == nullptr || d_scalars == nullptr)) return hipErrorInvalidValue;
+  char* ws = reinterpret_cast<char*>(d_workspace);
+  PhaseTimer pt{false, stream, {}, {}};
+  ZKP_CHECK(reset_buckets(cfg, L, ws, 0, stream));
+  if (len > 0) ZKP_CHECK(accumulate_part(cfg, L, ws, 0, d_points, d_scalars, len, true, stream, pt));
+  const u32* d_sums = nullptr;
+  ZKP_CHECK(reduce_buckets(cfg, L, ws, 0, stream, &d_sums, pt));
+  if (h_sums != nullptr)
+    ZKP_CHECK(hipMemcpyAsync(h_sums, d_sums, sizeof(G1XYZZ) * cfg.n_windows, hipMemcpyDeviceToHost, stream));
+  return hipSuccess;
+}
+
+// Result of an MSM from its window sums (as zkp_msm_device_enqueue copied
+// them): Horner over the windows on the host, affine into h_result[16].
+hipError_t zkp_msm_combine(unsigned int window, unsigned int target_windows, const void* h_sums,
+                           unsigned int* h_result) {
+  Config cfg;
+  if (!make_config(window, target_windows, &cfg)) return hipErrorInvalidValue;
+  if (h_sums == nullptr || h_result == nullptr) return hipErrorInvalidValue;
+  const G1XYZZ* p = reinterpret_cast<const G1XYZZ*>(h_sums);
+  std::vector<G1XYZZ> sums(p, p + cfg.n_windows);
+  const G1Affine a = combine_windows(cfg, sums).to_affine();
   memcpy(h_result, a.x.v, 32);
   memcpy(h_result + 8, a.y.v, 32);
   return hipSuccess;

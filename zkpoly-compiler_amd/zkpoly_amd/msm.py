@@ -88,6 +88,11 @@ def core_lib(libs: Libs) -> ctypes.CDLL:
         lib.zkp_g1_mul_generator.restype = ctypes.c_int
         lib.zkp_msm_config.argtypes = [ctypes.c_uint, ctypes.c_uint, P]
         lib.zkp_msm_config.restype = ctypes.c_int
+        lib.zkp_msm_device_enqueue.argtypes = [P, P, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, P,
+                                               ctypes.c_ulonglong, P, P]
+        lib.zkp_msm_device_enqueue.restype = ctypes.c_int
+        lib.zkp_msm_combine.argtypes = [ctypes.c_uint, ctypes.c_uint, P, P]
+        lib.zkp_msm_combine.restype = ctypes.c_int
         lib._zkp_typed = True
     return lib
 
@@ -171,6 +176,40 @@ def msm_device(libs: Libs, d_points: int, d_scalars: int, length: int, window: i
         "zkp_msm_device",
     )
     return out
+
+
+def msm_config(libs: Libs, window: int, target: int) -> dict:
+    """Derived MsmConfig values of the library (msm.cuh:51-70)."""
+    out = (ctypes.c_uint * 5)()
+    if not core_lib(libs).zkp_msm_config(window, target, out):
+        raise ValueError(f"invalid MSM configuration window={window} target={target}")
+    return dict(zip(("window", "actual_windows", "n_windows", "n_precompute", "buckets"), list(out)))
+
+
+class DeviceMsm:
+    """One device-resident MSM split at the host boundary: enqueue() puts the
+    device work and the window sums' copy-out on a stream (graph-capturable,
+    allocates nothing), result() combines the copied sums on the host."""
+
+    def __init__(self, libs: Libs, d_points: int, d_scalars: int, length: int, window: int, target: int | None = None):
+        self.lib = core_lib(libs)
+        self.d_points, self.d_scalars, self.length = d_points, d_scalars, length
+        self.window = window
+        self.target = target or -(-SCALAR_BITS // window)
+        n_windows = msm_config(libs, window, self.target)["n_windows"]
+        self.workspace = hip.DeviceBuffer(workspace_size(libs, length, window, self.target))
+        self.sums = hip.PinnedBuffer(n_windows * 128)
+
+    def enqueue(self, stream: hip.Stream) -> None:
+        hip.check(self.lib.zkp_msm_device_enqueue(self.d_points, self.d_scalars, self.length, self.window,
+                                                  self.target, self.workspace.addr, self.workspace.nbytes,
+                                                  stream.raw(), self.sums.addr), "zkp_msm_device_enqueue")
+
+    def result(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.uint64)
+        hip.check(self.lib.zkp_msm_combine(self.window, self.target, self.sums.addr, out.ctypes.data),
+                  "zkp_msm_combine")
+        return out
 
 
 def mul_generator(libs: Libs, d_scalars: int, n: int, d_out: int, stream: hip.Stream) -> None:
