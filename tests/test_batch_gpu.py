@@ -10,6 +10,7 @@ import pytest
 
 import oracle as orc
 from zkpoly_amd import fused
+from zkpoly_amd.field import from_mont
 from zkpoly_amd.batch import Config4Workload
 
 pytestmark = pytest.mark.gpu
@@ -67,6 +68,41 @@ def test_graph_replays_legacy_variant(libs, stream, kernel):
             x, out = w.results(i)
             assert np.array_equal(x, want[i][0]), (replay, i, w.log_lens[i])
             assert np.array_equal(out, want[i][1]), (replay, i)
+
+
+def msm_closed_form(w, j):
+    n = w.msm_lens[j]
+    total = from_mont(orc.to_int(orc.inner_product(w.msm_base_scalars[:n], w.msm_scalars[j])))
+    return list(orc.g1_mul_canonical(orc.generator(), total))
+
+
+def test_graph_with_msms_replays_bit_exact(libs, stream, kernel):
+    """NTTs, the fused kernel and MSMs of uneven sizes (2^10..2^16 points, each
+    its own window) in one graph; three replays with eager work between them,
+    every NTT, fused output and MSM result checked, and the MSMs equal to the
+    plain synchronous calls."""
+    w = Config4Workload(libs, stream, kernel, count=10, lo=6, hi=15, seed=11, msm_count=6, msm_lo=10, msm_hi=16)
+    print("msm sizes", w.msm_log_lens, "windows", [m.window for m in w.msms], flush=True)
+    want = [expected(w, i) for i in range(len(w.lens))]
+    want_msm = [msm_closed_form(w, j) for j in range(len(w.msms))]
+    graph = w.capture()
+    for replay in range(3):
+        w.reset_inputs()
+        for m in w.msms:
+            m.sums.view()[:] = 0
+        stream.sync()
+        graph.launch(stream)
+        stream.sync()
+        for i in range(len(w.lens)):
+            x, out = w.results(i)
+            assert np.array_equal(x, want[i][0]), (replay, i, w.log_lens[i])
+            assert np.array_equal(out, want[i][1]), (replay, i)
+        got = graph.msm_results()
+        for j in range(len(w.msms)):
+            assert list(got[j]) == want_msm[j], (replay, j, w.msm_log_lens[j])
+    plain = w.run_plain()
+    stream.sync()
+    assert [list(r) for r in plain] == want_msm
 
 
 def test_plain_path_matches(libs, stream, kernel):

@@ -1,13 +1,16 @@
 """Batched proof-step form: many NTTs of uneven sizes, each followed by a fused
-elementwise kernel, run as one hipGraph launch.
+elementwise kernel, and many MSMs of uneven sizes, run as one hipGraph launch.
 
 The plain form issues, per instance, ssip_ntt (core/src/ntt.rs:165-200) and
-the fused kernel (core/src/fused_kernels.rs:97-196) one call after another --
-what the reference runtime does instruction by instruction.  The batched form
-plans all transforms together (zkp_ntt_batch_create: one bit-reversal launch
-and one launch per pass round for the whole batch) and runs the fused
-expression over all instances with its multi-instance entry, and captures
-both in one hipGraph, so a batch costs one graph launch.
+the fused kernel (core/src/fused_kernels.rs:97-196) one call after another,
+and every MSM as its own synchronous call -- what the reference runtime does
+instruction by instruction.  The batched form plans all transforms together
+(zkp_ntt_batch_create: one bit-reversal launch and one launch per pass round
+for the whole batch), runs the fused expression over all instances with its
+multi-instance entry, enqueues every MSM's device work with its window sums'
+copy-out (zkp_msm_device_enqueue), and captures all of it in one hipGraph, so
+a batch costs one graph launch; the MSMs' host combines (a few microseconds
+each) follow the launch.
 """
 from __future__ import annotations
 
@@ -82,28 +85,41 @@ class FusedBatch:
         dv.upload(np.frombuffer(bytes(va), dtype=np.uint8))
         dm.upload(np.frombuffer(bytes(ma), dtype=np.uint8))
         dp.upload(prefix)
-        self._bufs = [dv, dm, dp]
+        local = hip.DeviceBuffer(self.kernel.local_bytes) if self.kernel.local_bytes else None
+        self._bufs = [dv, dm, dp, local]
 
     def run(self, stream: hip.Stream) -> None:
-        dv, dm, dp = self._bufs
-        hip.check(self.kernel.c_batched(dv.addr, dm.addr, dp.addr, self.count, self.total, True, stream.raw()),
+        dv, dm, dp, local = self._bufs
+        hip.check(self.kernel.c_batched(dv.addr, dm.addr, dp.addr, self.count, self.total, True, stream.raw(),
+                                        local.addr if local is not None else None),
                   self.kernel.meta.name + "_batched")
 
 
 class ProofStepBatch:
-    """NTT batch + fused batch captured as one hipGraph."""
+    """NTT batch + fused batch + MSMs captured as one hipGraph."""
 
-    def __init__(self, stream: hip.Stream, ntt: NttBatch, fused: FusedBatch):
-        self.ntt, self.fused = ntt, fused
+    def __init__(self, stream: hip.Stream, ntt: NttBatch, fused: FusedBatch, msms=()):
+        self.ntt, self.fused, self.msms = ntt, fused, list(msms)
 
         def fill(s):
             ntt.run(s)
             fused.run(s)
+            for m in self.msms:
+                m.enqueue(s)
 
         self.graph = hip.Graph(stream, fill)
 
     def launch(self, stream: hip.Stream) -> None:
         self.graph.launch(stream)
+
+    def msm_results(self) -> list:
+        """The MSMs' affine results once the launch has completed."""
+        return [m.result() for m in self.msms]
+
+
+def default_msm_window(log_n: int) -> int:
+    """Window for an MSM of 2^log_n points when the autotuner has no entry."""
+    return max(8, min(16, log_n - 4))
 
 
 class Config4Workload:
@@ -113,7 +129,7 @@ class Config4Workload:
     `tuning` (an autotune.Tuning) is installed before the transforms are planned."""
 
     def __init__(self, libs: Libs, stream: hip.Stream, kernel: FusedKernel, count: int = 64, lo: int = 10,
-                 hi: int = 22, seed: int = 4, tuning=None):
+                 hi: int = 22, seed: int = 4, tuning=None, msm_count: int = 0, msm_lo: int = 14, msm_hi: int = 18):
         from .field import mont_limbs, omega, random_elements
         from .ntt import gen_powers
 
@@ -170,21 +186,65 @@ class Config4Workload:
         self.ntt_batch = NttBatch(libs, self.x_ptrs, [self.tw[k][0].addr for k in self.log_lens], self.log_lens)
         self.fused_batch = FusedBatch(kernel, self.fused_vars, self.fused_muts, self.lens)
         self.total_elements = sum(self.lens)
+        self._setup_msms(libs, rng, seed, tuning, msm_count, msm_lo, msm_hi)
+
+    def _setup_msms(self, libs: Libs, rng, seed: int, tuning, count: int, lo: int, hi: int) -> None:
+        """`count` MSMs of 2^U{lo..hi} points: prefixes of one resident table of
+        bases P_i = k_i G, each with its own seeded scalars."""
+        from .field import random_elements
+        from .msm import DeviceMsm, mul_generator, msm_device
+
+        self._msm_device = msm_device
+        self.libs = libs
+        self.total_msm_points = 0
+        self.msm_log_lens = [int(x) for x in rng.integers(lo, hi + 1, size=count)]
+        self.msm_lens = [1 << k for k in self.msm_log_lens]
+        self.msms, self.msm_scalars = [], []
+        self.msm_base_scalars = None
+        if count == 0:
+            return
+        top = max(self.msm_lens)
+        self.msm_base_scalars = random_elements(top, seed * 1000 + 77)
+        kb = hip.DeviceBuffer(top * 32)
+        kb.upload(self.msm_base_scalars)
+        self.msm_bases = hip.DeviceBuffer(top * 64)
+        mul_generator(libs, kb.addr, top, self.msm_bases.addr, self.stream)
+        self._msm_scalar_bufs = []
+        for j, n in enumerate(self.msm_lens):
+            sc = random_elements(n, seed * 1000 + 100 + j)
+            b = hip.DeviceBuffer(n * 32)
+            b.upload(sc)
+            self.msm_scalars.append(sc)
+            self._msm_scalar_bufs.append(b)
+            k = self.msm_log_lens[j]
+            w = tuning.msm_window(k, default_msm_window(k)) if tuning is not None else default_msm_window(k)
+            self.msms.append(DeviceMsm(libs, self.msm_bases.addr, b.addr, n, w))
+        # one eager run before any capture: the first MSM on a device sets launch limits
+        for m in self.msms:
+            m.enqueue(self.stream)
+        self.stream.sync()
+        self.total_msm_points = sum(self.msm_lens)
 
     def reset_inputs(self) -> None:
         for d, h in zip(self.dev, self.host):
             d[0].upload(h[0])
 
-    def run_plain(self) -> None:
-        """One ssip_ntt + one fused call per instance, in order."""
+    def run_plain(self) -> list:
+        """One ssip_ntt + one fused call per instance, in order, then every MSM
+        as its own synchronous call; returns the MSM results."""
         s = self.stream
         for i, k in enumerate(self.log_lens):
             hip.check(self.ntt_lib.ssip_ntt(self.x_ptrs[i], self.tw[k][0].addr, k, s.raw(), 8, 8), "ssip_ntt")
             dv, dm = self.plain_args[i]
             hip.check(self.kernel.c_func(dv.addr, dm.addr, self.lens[i], True, None, s.raw()), "fused")
+        out = []
+        for m in self.msms:
+            out.append(self._msm_device(self.libs, m.d_points, m.d_scalars, m.length, m.window, m.target,
+                                        m.workspace, s))
+        return out
 
     def capture(self) -> ProofStepBatch:
-        return ProofStepBatch(self.stream, self.ntt_batch, self.fused_batch)
+        return ProofStepBatch(self.stream, self.ntt_batch, self.fused_batch, self.msms)
 
     def results(self, i: int) -> tuple[np.ndarray, np.ndarray]:
         n = self.lens[i]

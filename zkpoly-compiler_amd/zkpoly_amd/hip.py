@@ -262,7 +262,8 @@ hipStreamCaptureModeThreadLocal = 1
 class Graph:
     """A captured and instantiated hipGraph of the work enqueued by `fill(stream)`.
 
-    Only kernel launches may be captured here (no memsets or allocations).
+    Kernel launches and asynchronous copies may be captured here (no memsets,
+    which have given wrong results on replays, and no allocations).
     """
 
     def __init__(self, stream: Stream, fill):
