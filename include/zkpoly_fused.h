@@ -9,10 +9,15 @@
  *                     unsigned long long len, bool is_first,
  *                     void* local_buffer, hipStream_t stream);
  *
- * vars / mut_vars are DEVICE arrays; scalars are 1-element views.  Each
- * library also exports <name>_batched (instance k reads vars[k*NV..],
- * mut_vars[k*NM..] and elements [prefix[k], prefix[k+1])), <name>_arity and
- * <name>_mut_reads (one flag per mut_var: 1 when the kernel reads it).
+ * vars / mut_vars are DEVICE arrays; scalars are 1-element views.
+ * local_buffer holds at least <name>_local_buffer_bytes() bytes (the
+ * analogue of FusedOp::get_temp_buffer_size, kernel_gen.rs:136-146: the
+ * graph's scalar table and register spill slots; 0 for small graphs, which
+ * accept any buffer).  Each library also exports <name>_batched (instance k
+ * reads vars[k*NV..], mut_vars[k*NM..] and elements [prefix[k],
+ * prefix[k+1]); the same local_buffer, last), <name>_arity, <name>_mut_reads
+ * (one flag per mut_var: 1 when the kernel reads it) and
+ * <name>_local_buffer_bytes.
  */
 #ifndef ZKPOLY_FUSED_H
 #define ZKPOLY_FUSED_H
@@ -29,7 +34,9 @@ typedef hipError_t (*zkp_fused_fn)(const ConstPolyPtr* vars, const PolyPtr* mut_
                                    bool is_first, void* local_buffer, hipStream_t stream);
 typedef hipError_t (*zkp_fused_batched_fn)(const ConstPolyPtr* vars, const PolyPtr* mut_vars,
                                            const unsigned long long* prefix, unsigned int instances,
-                                           unsigned long long total, bool is_first, hipStream_t stream);
+                                           unsigned long long total, bool is_first, hipStream_t stream,
+                                           void* local_buffer);
+typedef unsigned long long (*zkp_fused_local_bytes_fn)(void);
 
 /* library: libzkp_fused_codegen.so */
 
@@ -40,6 +47,14 @@ typedef hipError_t (*zkp_fused_batched_fn)(const ConstPolyPtr* vars, const PolyP
  * -1 with a message in err. */
 int zkp_fused_codegen(const char* graph_json, const char* name, char* out, size_t out_cap, size_t* out_len,
                       char* err, size_t err_cap);
+
+/* Options of the kernels generated afterwards (process-wide, generator only):
+ * field elements held in registers per element before values spill to
+ * local_buffer (default 16, at least 3) and the waves per SIMD the compiler
+ * must keep (default 2; 0 leaves it to the compiler).  No counterpart in the
+ * reference (its register limit is derived inside FusedOp::new,
+ * kernel_gen.rs:88-106). */
+void zkp_fused_set_options(unsigned int reg_budget, unsigned int waves_per_eu);
 
 /* Generates every kernel into out_dir/<name>.hip and compiles out_dir/libfused_kernels.so
  * (the reference's xmake build of the generated sources, core/src/fused_kernels.rs:76-95). */

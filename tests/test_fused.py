@@ -225,7 +225,7 @@ def test_batched_entry_matches_single(libs, stream):
     dv.upload(np.frombuffer(bytes(va), dtype=np.uint8))
     dm.upload(np.frombuffer(bytes(ma), dtype=np.uint8))
     dp.upload(prefix)
-    hip.check(k.c_batched(dv.addr, dm.addr, dp.addr, len(lens), int(prefix[-1]), True, stream.raw()), "batched")
+    hip.check(k.c_batched(dv.addr, dm.addr, dp.addr, len(lens), int(prefix[-1]), True, stream.raw(), None), "batched")
     stream.sync()
     for (ob, w), n in zip(wants, lens):
         if n:

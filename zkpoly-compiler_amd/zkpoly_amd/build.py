@@ -136,6 +136,14 @@ def build_fused_demo() -> str:
     return fused.build_library(Libs(), fused.demo_kernels(), fused.DEMO_DIR)
 
 
+def build_fused_large() -> str:
+    """Prebuilt compiler-scale fused kernels (random 700-vertex graphs) for the tests."""
+    from . import fused
+    from .runtime import Libs
+
+    return fused.build_library(Libs(), fused.large_kernels(), fused.LARGE_DIR)
+
+
 def build_fused_pipeline() -> str:
     """Host-only driver of the pipelined fused kernel (HIP runtime API only)."""
     os.makedirs(LIB, exist_ok=True)
@@ -161,8 +169,21 @@ def build_serve() -> str:
     return out
 
 
+def build_pinned_pool() -> str:
+    """Host-only buddy allocator over pinned host slabs (HIP runtime API only)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libzkp_pinned_pool.so")
+    tmp = out + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run([shutil.which("g++") or "g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          f"-I{rocm}/include", os.path.join(CSRC, "runtime", "pinned_pool.cpp"), f"-L{rocm}/lib", "-lamdhip64",
+          "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def targets() -> list:
-    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_fused_pipeline]
+    return [build_ntt, build_poly, build_msm_core, build_fused_codegen, build_fused_pipeline, build_pinned_pool]
 
 
 def build_all(jobs: int | None = None) -> list[str]:
@@ -174,6 +195,7 @@ def build_all(jobs: int | None = None) -> list[str]:
         # demo library needs the generator
         futs = [ex.submit(build_msm_abi, w, t, False) for (w, t) in PREBUILT_MSM_CONFIGS]
         futs.append(ex.submit(build_fused_demo))
+        futs.append(ex.submit(build_fused_large))
         futs.append(ex.submit(build_serve))
         out += [f.result() for f in futs]
     return out

@@ -178,6 +178,73 @@ def graph_pipelined_coef() -> str:
     return g.to_json()
 
 
+def random_graph(seed: int, vertices: int = 600, poly_inputs: int = 12, scalar_inputs: int = 6,
+                 inplace: int = 2, poly_outputs: int = 16, scalar_outputs: int = 3, coef: bool = False) -> str:
+    """A seeded ArithGraph of about `vertices` vertices with every operation the
+    generator accepts (Pp / Sp / Ss add, sub, mul, Sp SubBy, Neg of both kinds,
+    Pow), in-place outputs over mutable inputs and scalar outputs, in Lagrange
+    or coefficient form -- the size a compiler's fusion pass produces for a
+    proof's constraint expressions.  Operands are drawn mostly from recent
+    values (long dependent chains) and sometimes from anywhere (long live
+    ranges).  vars: the const inputs in order (polynomials, then scalars);
+    mut_vars: the in-place outputs, then the polynomial outputs, then the
+    scalar outputs."""
+    import random
+
+    rng = random.Random(seed)
+    g = GraphBuilder("Coef" if coef else "Lagrange")
+    mut_in = [g.input(mutable=True) for _ in range(inplace)]
+    polys = mut_in + [g.input() for _ in range(poly_inputs)]
+    scalars = [g.input(scalar=True) for _ in range(scalar_inputs)]
+
+    unused = []  # computed values nothing reads yet: operands and outputs prefer them
+
+    def pick(pool, kind):
+        cands = [u for u in unused if (u in kind_of and kind_of[u] == kind)]
+        if cands and rng.random() < 0.5:
+            v = cands[-1] if rng.random() < 0.7 else rng.choice(cands)
+        elif rng.random() < 0.7:
+            v = pool[-1 - min(len(pool) - 1, int(rng.expovariate(0.25)))]
+        else:
+            v = rng.choice(pool)
+        if v in unused:
+            unused.remove(v)
+        return v
+
+    kind_of = {}
+
+    def made(v, kind):
+        kind_of[v] = kind
+        unused.append(v)
+        return v
+
+    budget = vertices - len(g.vertices) - inplace - poly_outputs - scalar_outputs
+    for _ in range(max(0, budget)):
+        r = rng.random()
+        if r < 0.12:
+            scalars.append(made(g.bin("Ss", rng.choice(("Add", "Sub", "Mul")), pick(scalars, "s"), pick(scalars, "s")), "s"))
+        elif r < 0.15:
+            scalars.append(made(g.neg(pick(scalars, "s"), scalar=True), "s"))
+        elif r < 0.50:
+            polys.append(made(g.bin("Pp", rng.choice(("Add", "Sub", "Mul", "Mul")), pick(polys, "p"), pick(polys, "p")),
+                              "p"))
+        elif r < 0.85:
+            polys.append(made(g.bin("Sp", rng.choice(("Add", "Sub", "SubBy", "Mul")), pick(scalars, "s"),
+                                    pick(polys, "p")), "p"))
+        elif r < 0.93:
+            polys.append(made(g.neg(pick(polys, "p")), "p"))
+        else:
+            polys.append(made(g.pow(pick(polys, "p"), rng.randint(2, 9)), "p"))
+    computed = polys[len(mut_in) + poly_inputs:]
+    for m in mut_in:
+        g.output(pick(computed, "p"), in_node=m)
+    for _ in range(poly_outputs):
+        g.output(pick(computed, "p"))
+    for _ in range(scalar_outputs):
+        g.output(pick(scalars, "s"), scalar=True)
+    return g.to_json()
+
+
 def demo_kernels() -> list[tuple[str, str]]:
     return [
         (graph_ab_gamma_c_minus_z(), "fused_arith_abgcz"),
@@ -190,6 +257,21 @@ def demo_kernels() -> list[tuple[str, str]]:
 
 
 DEMO_DIR = os.path.join(PACKAGE_ROOT, "lib", "fused_demo")
+
+# Compiler-scale graphs kept prebuilt for the tests (random_graph, Lagrange and
+# coefficient form), in a library of their own.
+LARGE_DIR = os.path.join(PACKAGE_ROOT, "lib", "fused_large")
+LARGE_SEEDS = (101, 202)
+LARGE_VERTICES = 800
+
+
+def large_kernel_name(seed: int, coef: bool) -> str:
+    return f"fused_arith_large_{seed}_{'coef' if coef else 'lagrange'}"
+
+
+def large_kernels() -> list[tuple[str, str]]:
+    return [(random_graph(LARGE_SEEDS[0], LARGE_VERTICES, coef=False), large_kernel_name(LARGE_SEEDS[0], False)),
+            (random_graph(LARGE_SEEDS[1], LARGE_VERTICES, coef=True), large_kernel_name(LARGE_SEEDS[1], True))]
 
 
 def _codegen_lib(libs: Libs) -> ctypes.CDLL:
@@ -208,6 +290,15 @@ def _codegen_lib(libs: Libs) -> ctypes.CDLL:
 
 class CodegenError(ValueError):
     pass
+
+
+def set_codegen_options(libs: Libs, reg_budget: int = 16, waves_per_eu: int = 2) -> None:
+    """Register budget (field elements per element before spilling to
+    local_buffer) and waves per SIMD of the kernels generated afterwards."""
+    lib = _codegen_lib(libs)
+    lib.zkp_fused_set_options.argtypes = [ctypes.c_uint, ctypes.c_uint]
+    lib.zkp_fused_set_options.restype = None
+    lib.zkp_fused_set_options(reg_budget, waves_per_eu)
 
 
 def codegen(libs: Libs, graph_json: str, name: str) -> str:
@@ -261,6 +352,14 @@ def arg_buffer_bytes(meta: FusedKernelMeta) -> int:
     return (meta.num_vars + meta.num_mut_vars) * POLYPTR_BYTES
 
 
+def local_buffer_bytes(meta: FusedKernelMeta) -> int:
+    """Size of the kernel's local buffer (the reference's FusedOp::get_temp_buffer_size,
+    kernel_gen.rs:136-146): the generated library reports it -- the table of the
+    graph's scalar values and the register spill slots of its capped grid; 0
+    when the kernel needs none (the buffer may then be empty)."""
+    return int(getattr(_load(meta), meta.name + "_local_buffer_bytes")())
+
+
 def _load(meta: FusedKernelMeta) -> ctypes.CDLL:
     path = os.path.join(meta.lib_path, LIB_NAME)
     if not os.path.exists(path):
@@ -272,8 +371,11 @@ def _load(meta: FusedKernelMeta) -> ctypes.CDLL:
     f.restype = ctypes.c_int
     fb = getattr(lib, meta.name + "_batched")
     fb.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_ulonglong,
-                   ctypes.c_bool, ctypes.c_void_p]
+                   ctypes.c_bool, ctypes.c_void_p, ctypes.c_void_p]
     fb.restype = ctypes.c_int
+    lb = getattr(lib, meta.name + "_local_buffer_bytes")
+    lb.argtypes = []
+    lb.restype = ctypes.c_ulonglong
     ar = getattr(lib, meta.name + "_arity")
     ar.argtypes = [ctypes.c_void_p]
     out = (ctypes.c_uint * 2)()
@@ -297,9 +399,10 @@ class FusedKernel(RegisteredFunction):
         self.lib = _load(meta)
         self.c_func = getattr(self.lib, meta.name)
         self.c_batched = getattr(self.lib, meta.name + "_batched")
+        self.local_bytes = int(getattr(self.lib, meta.name + "_local_buffer_bytes")())
 
     def get_fn(self) -> Function:
-        meta, c_func = self.meta, self.c_func
+        meta, c_func, local_bytes = self.meta, self.c_func, self.local_bytes
 
         def f(mut_var, var, gpu_mapping=None):
             assert meta.num_vars == len(var) - 1
@@ -308,6 +411,7 @@ class FusedKernel(RegisteredFunction):
             arg_buffer, local_buffer = mut_var[0], mut_var[1]
             assert isinstance(arg_buffer, GpuBuffer) and isinstance(local_buffer, GpuBuffer)
             assert arg_buffer.size >= arg_buffer_bytes(meta)
+            assert local_buffer.size >= local_bytes, "local buffer too small"
             length = 0
             for v in list(var[1:]) + list(mut_var[2:]):
                 if isinstance(v, ScalarArray):
@@ -384,10 +488,13 @@ class PipelinedFusedKernel(RegisteredFunction):
         p, q, parts = pm.num_mut_scalars, pm.num_scalars, pm.divide_parts
         poly_reads = self.mut_reads[p:]  # mut_vars are [mut scalars..., mut polys...]
 
+        local_bytes = self.kernel.local_bytes
+
         def f(mut_var, var, gpu_mapping=None):
             arg_buffer, local_buffer = mut_var[0], mut_var[1]
             assert isinstance(arg_buffer, GpuBuffer) and isinstance(local_buffer, GpuBuffer)
             assert arg_buffer.size >= pipelined_arg_buffer_bytes(meta)
+            assert local_buffer.size >= local_bytes, "local buffer too small"
             rest = mut_var[2:]
             assert (len(rest) - 2 * p) % 4 == 0
             assert (len(var) - 2 * q) % 3 == 0
