@@ -295,32 +295,45 @@ __device__ __forceinline__ void lds_put(uint4* l0, uint4* l1, uint32_t p, const 
 constexpr uint32_t HEAD_BUF = HEAD_W * HEAD_PITCH;  // one 32 x 33 tile plane
 
 // Two tile buffers per plane (x at M at offset 0, x at rev M at HEAD_BUF) and
-// the head's twiddles r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1).
+// the head's twiddles r[i] = w^(i * 2^(log_n - t)), i < 2^(t - 1).  The second
+// plane starts 128 bytes into the LDS bank cycle relative to the first, so the
+// two lanes that move the two halves of one element hit different banks.
 struct HeadLds {
-  uint4 p0[2 * HEAD_BUF], p1[2 * HEAD_BUF];
+  uint4 p0[2 * HEAD_BUF];
+  uint4 pad[8];
+  uint4 p1[2 * HEAD_BUF];
   uint4 r0[HEAD_W / 2], r1[HEAD_W / 2];
 };
+static_assert((2 * HEAD_BUF * 16 + 128) % 256 == 128, "planes half a bank cycle apart");
 
-// Per-thread staging of the head's global loads: element e = j * NT +
-// threadIdx.x of the tile at M (a) and of the tile at rev M (b), j < 1024 / NT.
+// The head moves tiles between HBM and LDS as 16-byte halves of elements:
+// half h of a tile is plane (h & 1) of element h >> 1, and consecutive lanes
+// take consecutive halves, so one wave instruction reads or writes 1 KiB of a
+// tile row (32 elements, contiguous in memory) and every 128-byte line it
+// touches is filled by that one instruction.
+__device__ __forceinline__ uint4* head_plane(HeadLds& L, uint32_t plane) { return plane ? L.p1 : L.p0; }
+__device__ __forceinline__ const uint4* head_plane(const HeadLds& L, uint32_t plane) { return plane ? L.p1 : L.p0; }
+
+// Per-thread staging of the head's global loads: half h = j * NT +
+// threadIdx.x of the tile at M (a) and of the tile at rev M (b), j < 2048 / NT.
 struct HeadStaged {
-  uint4 a0, a1, b0, b1;
+  uint4 a, b;
 };
 
-// Load j of the thread (element e = j * NT + threadIdx.x of the tile at M and
-// of the tile at rev M) for a workgroup of NT threads.  The loads are
-// unconditional: the index is clamped for t < 5, and x at M is read twice when
-// M == rev M.  Callers keep the results in named values so the staging stays
-// in registers.
+// Load j of the thread for a workgroup of NT threads.  The loads are
+// unconditional: the half index is clamped for t < 5, and x at M is read twice
+// when M == rev M.  Callers keep the results in named values so the staging
+// stays in registers.
 template <int NT>
 __device__ __forceinline__ HeadStaged head_fetch(const PolyView& v, uint32_t t, uint32_t shift_a, uint64_t M,
                                                  uint64_t Mr, uint32_t j) {
   const uint32_t W = 1u << t;
-  const uint32_t e = (threadIdx.x + j * NT) & (W * W - 1);
+  const uint32_t h = (threadIdx.x + j * NT) & (2 * W * W - 1);
+  const uint32_t e = h >> 1, plane = h & 1;
   const uint32_t a = e >> t, b = e & (W - 1);
   const uint4* s = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
   const uint4* q = reinterpret_cast<const uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
-  return HeadStaged{s[0], s[1], q[0], q[1]};
+  return HeadStaged{s[plane], q[plane]};
 }
 
 // Staged load j into the two LDS tile buffers (x at M at offset 0, x at rev M
@@ -328,15 +341,13 @@ __device__ __forceinline__ HeadStaged head_fetch(const PolyView& v, uint32_t t, 
 template <int NT>
 __device__ __forceinline__ void head_place(HeadLds& L, uint32_t t, bool pair, uint32_t j, const HeadStaged& g) {
   const uint32_t W = 1u << t;
-  const uint32_t e = threadIdx.x + j * NT;
-  if (e < W * W) {
+  const uint32_t h = threadIdx.x + j * NT;
+  if (h < 2 * W * W) {
+    const uint32_t e = h >> 1;
     const uint32_t p = (e >> t) * HEAD_PITCH + (e & (W - 1));
-    L.p0[p] = g.a0;
-    L.p1[p] = g.a1;
-    if (pair) {
-      L.p0[HEAD_BUF + p] = g.b0;
-      L.p1[HEAD_BUF + p] = g.b1;
-    }
+    uint4* dst = head_plane(L, h & 1);
+    dst[p] = g.a;
+    if (pair) dst[HEAD_BUF + p] = g.b;
   }
 }
 
@@ -405,7 +416,12 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
           const Fr x0 = Fr::reduce_2p(x[u][0]), x2 = Fr::reduce_2p(x[u][2]);
           const Fr a0 = Fr::add_raw(x0, t1), a1 = Fr::sub_2p(x0, t1);
           const Fr a2 = Fr::add_raw(x2, t3), a3 = Fr::sub_2p(x2, t3);
-          const Fr t2 = Fr::mul_lazy(a2, w2);
+          // at m = 0 the second stage's first twiddle is w^0 too: a reduction instead of a product
+          Fr t2;
+          if (skip1)
+            t2 = Fr::reduce_2p(a2);
+          else
+            t2 = Fr::mul_lazy(a2, w2);
           const Fr t4 = Fr::mul_lazy(a3, w3);
           const Fr a0n = Fr::reduce_2p(a0), a1n = Fr::reduce_2p(a1);
           x[u][0] = Fr::add_raw(a0n, t2);
@@ -420,7 +436,8 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
           }
           const Fr a0 = x[u][0] + t1, a1 = x[u][0] - t1;
           const Fr a2 = x[u][2] + t3, a3 = x[u][2] - t3;
-          const Fr t2 = a2 * w2;
+          Fr t2 = a2;
+          if (!skip1) t2 = a2 * w2;
           const Fr t4 = a3 * w3;
           x[u][0] = a0 + t2;
           x[u][2] = a0 - t2;
@@ -491,23 +508,20 @@ __device__ __forceinline__ void head_stages(HeadLds& L, uint32_t t, bool pair) {
   }
 }
 
-// Both y tiles back in place (y at M from the buffer at y0, y at Mr from 0).
+// Both y tiles back in place (y at M from the buffer at y0, y at Mr from 0),
+// as 16-byte halves on consecutive lanes.
 template <int NT>
 __device__ __forceinline__ void head_store(const HeadLds& L, const PolyView& v, uint32_t t, uint32_t shift_a,
                                            uint64_t M, uint64_t Mr, bool pair) {
   const uint32_t W = 1u << t;
   const uint32_t y0 = pair ? HEAD_BUF : 0;
-  for (uint32_t e = threadIdx.x; e < W * W; e += NT) {
+  for (uint32_t h = threadIdx.x; h < 2 * W * W; h += NT) {
+    const uint32_t e = h >> 1, plane = h & 1;
     const uint32_t a = e >> t, b = e & (W - 1);
     const uint32_t p = rev_bits(b, t) * HEAD_PITCH + rev_bits(a, t);
-    uint4* d = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b));
-    d[0] = L.p0[y0 + p];
-    d[1] = L.p1[y0 + p];
-    if (pair) {
-      uint4* d2 = reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b));
-      d2[0] = L.p0[p];
-      d2[1] = L.p1[p];
-    }
+    const uint4* src = head_plane(L, plane);
+    reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (M << t) | b))[plane] = src[y0 + p];
+    if (pair) reinterpret_cast<uint4*>(v.elem(((uint64_t)a << shift_a) | (Mr << t) | b))[plane] = src[p];
   }
 }
 
@@ -521,12 +535,12 @@ __device__ __forceinline__ void head_tile(HeadLds& L, const PolyView& v, uint32_
   if (M > Mr) return;  // the pair is handled by block Mr (uniform per block)
   const bool pair = M != Mr;
   const uint32_t shift_a = log_n - t;
-  const HeadStaged g0 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 0), g1 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 1),
-                   g2 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 2), g3 = head_fetch<THREADS>(v, t, shift_a, M, Mr, 3);
-  head_place<THREADS>(L, t, pair, 0, g0);
-  head_place<THREADS>(L, t, pair, 1, g1);
-  head_place<THREADS>(L, t, pair, 2, g2);
-  head_place<THREADS>(L, t, pair, 3, g3);
+  static_assert(2 * HEAD_W * HEAD_W == 8 * THREADS, "eight staged loads per thread");
+  HeadStaged g[8];
+#pragma unroll
+  for (uint32_t j = 0; j < 8; j++) g[j] = head_fetch<THREADS>(v, t, shift_a, M, Mr, j);
+#pragma unroll
+  for (uint32_t j = 0; j < 8; j++) head_place<THREADS>(L, t, pair, j, g[j]);
   __syncthreads();
   head_stages<LAZY, THREADS>(L, t, pair);
   head_store<THREADS>(L, v, t, shift_a, M, Mr, pair);
@@ -578,23 +592,19 @@ __device__ __forceinline__ void head_walk(HeadLds& L, const PolyView& v, uint32_
   };
   uint64_t k = next(0);
   if (k >= K) return;
-  static_assert(HEAD_W * HEAD_W == 4 * HEAD_THREADS, "four staged loads per thread");
-  HeadStaged g0, g1, g2, g3;
+  static_assert(2 * HEAD_W * HEAD_W == 8 * HEAD_THREADS, "eight staged loads per thread");
+  HeadStaged stg[8];
   auto fetch = [&](uint64_t M) {
     const uint64_t Mr = rev(M);
-    g0 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 0);
-    g1 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 1);
-    g2 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 2);
-    g3 = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, 3);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) stg[j] = head_fetch<HEAD_THREADS>(v, t, shift_a, M, Mr, j);
   };
   fetch(item(k));
   for (;;) {
     const uint64_t M = item(k), Mr = rev(M);
     const bool pair = M != Mr;
-    head_place<HEAD_THREADS>(L, t, pair, 0, g0);
-    head_place<HEAD_THREADS>(L, t, pair, 1, g1);
-    head_place<HEAD_THREADS>(L, t, pair, 2, g2);
-    head_place<HEAD_THREADS>(L, t, pair, 3, g3);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) head_place<HEAD_THREADS>(L, t, pair, j, stg[j]);
     __syncthreads();
     const uint64_t kn = next(k + 1);
     if (kn < K) fetch(item(kn));
