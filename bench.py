@@ -34,6 +34,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Cores this process may run on (its affinity set), and what the OS reports."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +213,7 @@ def suite(libs, stream, steps):
         for name, fn in (("ntt", lambda: w.ntt()), ("intt", lambda: w.ntt(True)), ("coset_ntt", coset),
                          ("recompute_ntt", recompute)):
             ms = time_loop(stream, fn, steps, 2)
-            out[f"{name}_2^{k}"] = {"ms": round(ms, 4), "gelem_s": round(n / ms / 1e6, 3)}
+            out[f"{name}_2^{k}"] = suite_entry(n, k, ms, name)
         del w, pw, pqb, omb
     # 4x extended coset of a 2^22 polynomial: zero-pad to 2^24, coset g = 7, NTT on 2^24
     w = Ntt(libs, stream, 24, seed=4422)
@@ -222,8 +231,26 @@ def suite(libs, stream, steps):
         w.ntt()
 
     ms = time_loop(stream, ext, steps, 2)
-    out["extended_coset_4x_2^22"] = {"ms": round(ms, 4), "gelem_s_out": round(n / ms / 1e6, 3)}
+    e = suite_entry(n, 24, ms, "coset_ntt")
+    # algorithmic bytes of the extended transform: the 2^22 coefficients in, 2^24 values out
+    e["hbm_gbs"] = round(32 * ((1 << 22) + n) / (ms * 1e-3) / 1e9, 1)
+    e["hbm_frac"] = round(e["hbm_gbs"] / HBM_PEAK_GBS, 4)
+    e["gelem_s_out"] = e.pop("gelem_s")
+    out["extended_coset_4x_2^22"] = e
     return out
+
+
+def suite_entry(n, k, ms, name):
+    """ms, Gelem/s, algorithmic HBM rate (one read and one write of every
+    element) and integer-multiply use (136 v_mad_u64_u32 per product: n/2
+    products per stage, plus one per element for the coset's powers or the
+    inverse's n^-1 scaling) of one transform."""
+    mads = MADS_PER_MODMUL * (n // 2) * k
+    if name in ("intt", "coset_ntt"):
+        mads += MADS_PER_MODMUL * n
+    gbs = 64 * n / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "gelem_s": round(n / ms / 1e6, 3), "hbm_gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "int_frac": round(mads / (ms * 1e-3) / INT_MUL_PEAK, 4)}
 
 
 def msm_leg(libs, stream, log_n, steps, with_cpu):
@@ -271,6 +298,8 @@ def msm_leg(libs, stream, log_n, steps, with_cpu):
     best = min((v["ms_median"], k) for k, v in out["windows"].items() if "zeros" not in k)
     out["best_window"] = best[1]
     out["ms"] = best[0]
+    log("[bench] msm symbol end to end (host points and scalars in pinned memory)")
+    out["reference_abi"] = msm_abi_leg(libs, pb, ss, n, steps, results[(16, "uniform")])
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -281,15 +310,72 @@ def msm_leg(libs, stream, log_n, steps, with_cpu):
             total = from_mont(orc.to_int(orc.inner_product(ks, arr)))
             want[tag] = list(orc.g1_mul_canonical(orc.generator(), total))
         out["bit_exact"] = all(list(r) == want[tag] for (w, tag), r in results.items())
-        threads = min(16, os.cpu_count() or 1)
+        threads, reported = host_cores()
         bases = pb.download(np.uint64, n * 8).reshape(n, 8)
         log(f"[bench] cpu baseline: oracle best_multiexp 2^{log_n} on {threads} threads")
         t = time.perf_counter()
         cpu = orc.best_multiexp(ss, bases, threads)
         cpu_s = time.perf_counter() - t
-        out["cpu_baseline"] = {"ms": round(cpu_s * 1e3, 2), "cores": threads, "kind": "port",
-                               "sample": f"best_multiexp restatement on the same 2^{log_n} uniform instance"}
+        out["cpu_baseline"] = {"ms": round(cpu_s * 1e3, 2), "cores": threads, "kind": "port", "os_cpu_count": reported,
+                               "sample": f"best_multiexp restatement on the same 2^{log_n} uniform instance, "
+                                         f"all {threads} cores of the process's affinity set"}
+        m1 = min(n, 1 << 17)
+        t = time.perf_counter()
+        orc.best_multiexp(ss[:m1], bases[:m1], 1)
+        cpu1_s = time.perf_counter() - t
+        out["cpu_baseline_1core"] = {"ms": round(cpu1_s * 1e3, 2), "cores": 1, "kind": "port", "points": m1,
+                                     "sample": f"best_multiexp on the first 2^{m1.bit_length() - 1} points, one core"}
         out["bit_exact"] = out["bit_exact"] and list(cpu) == want["uniform"]
+    return out
+
+
+def msm_abi_leg(libs, pb, ss, n, steps, expected):
+    """The drop-in msm symbol end to end: points and scalars in host memory
+    from the runtime's pinned CPU pool, copied in by the library on every call
+    (as the reference's msm does), the affine result back on the host.  One
+    batch with the whole range in one part, the same in four staged parts
+    (copies overlapping the accumulation), the planner's choice, and four
+    batches sharing one pass over the points."""
+    from zkpoly_amd import hip
+    from zkpoly_amd.msm import MSM, MsmConfig
+    from zkpoly_amd.msm_plan import plan_msm
+    from zkpoly_amd.pool import CpuMemoryPool
+    from zkpoly_amd.runtime import GpuBuffer, Point, PointArray, ScalarArray
+
+    pool = CpuMemoryPool(22, 64, libs)  # 256 MiB pinned slabs
+    pts = pool.allocate(n * 64)
+    pool.view(pts, n * 64)[:] = pb.download(np.uint64, n * 8)
+    sc = pool.allocate(n * 32)
+    pool.view(sc, n * 32)[:] = ss.reshape(-1)
+    planned = plan_msm(n, 1, 288 << 30, libs)
+    cases = {
+        "w16_one_part": (MsmConfig(16, 16), 1),
+        "w16_four_parts_staged": (MsmConfig(16, 16, parts=4, stage_scalars=2, stage_points=2), 1),
+        "planned": (planned, 1),
+        "w16_four_batches_one_run": (MsmConfig(16, 16, batch_per_run=4), 4),
+    }
+    out = {}
+    for name, (cfg, batches) in cases.items():
+        op = MSM(libs, cfg)
+        size = op.get_buffer_size(n)[0]
+        buf = hip.DeviceBuffer(size)
+        res = [Point() for _ in range(batches)]
+        f = op.get_fn().f
+        args = ([GpuBuffer(buf.addr, size)] + res, [PointArray(pts, n)] + [ScalarArray(n, sc, "cpu")] * batches)
+        f(*args, lambda c: c)
+        times = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            f(*args, lambda c: c)
+            times.append((time.perf_counter() - t) * 1e3)
+        times.sort()
+        out[name] = {"window": cfg.window_size, "target_window": cfg.target_window, "parts": cfg.parts,
+                     "batch_per_run": cfg.batch_per_run, "stages": [cfg.stage_scalars, cfg.stage_points],
+                     "batches": batches, "ms_median": round(times[len(times) // 2], 3),
+                     "ms_per_batch": round(times[len(times) // 2] / batches, 3),
+                     "device_bytes": size, "bit_exact_vs_device": all(list(r.data) == list(expected) for r in res)}
+        del buf
+    pool.destroy()
     return out
 
 
@@ -320,6 +406,37 @@ def batched_leg(libs, stream, steps, with_cpu):
         "graph_launches_batched": 1,
         "autotune_ntt_min_col_log": {str(k): v for k, v in sorted(tuning.ntt.items())},
     }
+    # the same 64 transforms and fused kernels plus 16 MSMs of 2^12..2^20 points in the one graph;
+    # plain: every MSM its own synchronous call (device work, window sums out, host combine)
+    log("[bench] batched form with MSMs (64 NTT + fused + 16 MSMs)")
+    wm = Config4Workload(libs, stream, kernel, count=64, lo=10, hi=22, seed=4, tuning=tuning, msm_count=16,
+                         msm_lo=12, msm_hi=20)
+    gm = wm.capture()
+    plain_m = time_loop(stream, wm.run_plain, steps, 2)
+
+    def batched_with_msms():
+        gm.launch(stream)
+        stream.sync()
+        gm.msm_results()
+
+    batched_m = time_loop(stream, batched_with_msms, steps, 2)
+    work = wm.total_elements + wm.total_msm_points
+    res_plain = wm.run_plain()
+    stream.sync()
+    gm.launch(stream)
+    stream.sync()
+    out["with_msms"] = {
+        "msm_log_sizes": wm.msm_log_lens,
+        "msm_windows": [m.window for m in wm.msms],
+        "ntt_elements": wm.total_elements,
+        "msm_points": wm.total_msm_points,
+        "plain_ms": round(plain_m, 4),
+        "batched_ms": round(batched_m, 4),
+        "plain_gelem_s": round(work / plain_m / 1e6, 4),
+        "batched_gelem_s": round(work / batched_m / 1e6, 4),
+        "speedup": round(plain_m / batched_m, 4),
+        "msm_results_equal_plain": [list(a) for a in res_plain] == [list(b) for b in gm.msm_results()],
+    }
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -338,7 +455,7 @@ def batched_leg(libs, stream, steps, with_cpu):
             z = np.broadcast_to(w.gamma_z[1], (n, 4)).copy()
             want = orc.vec("orc_vec_sub", orc.FR, orc.vec("orc_vec_add", orc.FR, orc.vec("orc_vec_mul", orc.FR, a, b),
                                                           orc.vec("orc_vec_mul", orc.FR, g, c)), z)
-            ok = ok and np.array_equal(x, orc.ntt(xh, w.log_lens[i], min(16, os.cpu_count() or 1)))
+            ok = ok and np.array_equal(x, orc.ntt(xh, w.log_lens[i], host_cores()[0]))
             ok = ok and np.array_equal(o, want)
         out["bit_exact_spot_check"] = bool(ok)
     return out
@@ -364,7 +481,7 @@ def serving_leg(libs, stream, rates, count, with_cpu):
         srv.run(reqs)
         out["rates"][str(rate)] = latency_stats(reqs, trace.jobs)
         if with_cpu:
-            threads = min(16, os.cpu_count() or 1)
+            threads = host_cores()[0]
             for i, j in enumerate(trace.jobs):
                 key = (j.kind, j.log_n, j.input_id, rate)
                 if key not in want:
@@ -607,6 +724,39 @@ def pipeline_leg(libs, stream, log_n, parts, steps, with_cpu=False, pinned=None)
     }
 
 
+def fused_device_leg(libs, stream, log_n, steps):
+    """The generated fused kernel a*b + gamma*c - z over 2^log_n device-resident
+    elements: three polynomials read and one written per element (128 B), two
+    scalars; HBM rate against the 8 TB/s peak."""
+    from zkpoly_amd import fused, hip
+    from zkpoly_amd.field import random_elements
+    from zkpoly_amd.runtime import PolyPtr
+
+    n = 1 << log_n
+    bufs = [hip.DeviceBuffer(n * 32) for _ in range(4)]
+    for i in range(3):
+        bufs[i].upload(random_elements(n, 700 + i))
+    dsc = [hip.DeviceBuffer(32) for _ in range(2)]
+    for i in range(2):
+        dsc[i].upload(random_elements(1, 710 + i))
+    plain = fused.FusedKernel(libs, fused.FusedKernelMeta("fused_arith_pipe_abgcz", 5, 1, fused.DEMO_DIR))
+    args = (PolyPtr * 6)(PolyPtr(dsc[0].addr, 1, 0, 0, 1), PolyPtr(dsc[1].addr, 1, 0, 0, 1),
+                         *[PolyPtr(bufs[i].addr, n, 0, 0, n) for i in range(4)])
+    dargs = hip.DeviceBuffer(ctypes.sizeof(args))
+    h = hip._lib()
+    hip.check(h.hipMemcpy(ctypes.c_void_p(dargs.addr), ctypes.addressof(args), ctypes.sizeof(args),
+                          hip.hipMemcpyHostToDevice), "args")
+
+    def run():
+        hip.check(plain.c_func(dargs.addr, dargs.addr + 5 * 40, n, True, None, stream.raw()), "fused kernel")
+
+    ms = time_loop(stream, run, steps, 3)
+    gbs = 128 * n / (ms * 1e-3) / 1e9
+    return {"log_n": log_n, "kernel": "fused_arith_pipe_abgcz", "ms": round(ms, 4), "bytes_alg": 128 * n,
+            "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "int_frac": round(2 * MADS_PER_MODMUL * n / (ms * 1e-3) / INT_MUL_PEAK, 4)}
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch from the committed PMC summary, if one exists."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -731,7 +881,9 @@ def main():
         host_in = w.host
         del w
         log("[bench] config-2 suite")
-        line["suite"] = suite(libs, stream, max(5, args.steps // 2))
+        line["suite"] = suite(libs, stream, max(10, args.steps))
+        log("[bench] fused a*b + gamma*c - z, device-resident 2^24")
+        line["fused_device"] = fused_device_leg(libs, stream, 24, max(10, args.steps))
         log("[bench] libpoly.so at 2^24")
         line["poly"] = poly_leg(libs, stream, 24, max(3, args.steps // 4), not args.no_cpu)
         log("[bench] pipelined fused kernel over host memory (2^24, 8 chunks)")
@@ -750,18 +902,30 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
 
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline: oracle best_fft 2^{k} on {threads} threads")
+        usable, reported = host_cores()
+        log(f"[bench] cpu baseline: oracle best_fft 2^{k} on {usable} threads")
         t = time.perf_counter()
-        want = orc.ntt(host_in, k, threads)
+        want = orc.ntt(host_in, k, usable)
         cpu_s = time.perf_counter() - t
         line["cpu_baseline"] = {
             "value": round(n / cpu_s / 1e9, 5),
             "unit": "Gelem/s",
-            "cores": threads,
+            "cores": usable,
             "kind": "port",
-            "sample": f"one forward NTT of the same 2^{k} input (best_fft restatement, 4x64-bit Montgomery)",
+            "sample": f"one forward NTT of the same 2^{k} input (best_fft restatement, 4x64-bit Montgomery), "
+                      f"all {usable} cores of the process's affinity set (os.cpu_count() = {reported})",
             "seconds": round(cpu_s, 3),
+            "os_cpu_count": reported,
+        }
+        k1 = min(k, 21)
+        log(f"[bench] cpu baseline: oracle best_fft 2^{k1} on 1 thread")
+        t = time.perf_counter()
+        orc.ntt(host_in[: 1 << k1], k1, 1)
+        cpu1_s = time.perf_counter() - t
+        line["cpu_baseline_1core"] = {
+            "value": round((1 << k1) / cpu1_s / 1e9, 5), "unit": "Gelem/s", "cores": 1, "kind": "port",
+            "sample": f"one forward NTT of 2^{k1} elements (the first 2^{k1} of the same input) on one core",
+            "seconds": round(cpu1_s, 3),
         }
         line["bit_exact"] = bool(np.array_equal(gpu_out, want))
     print(json.dumps(line), flush=True)
