@@ -761,6 +761,67 @@ __global__ void __launch_bounds__(THREADS)
   if (threadIdx.x == 0) sy[0].store(out + (size_t)blockIdx.x * 32);
 }
 
+// The same level with a quad of lanes per point operation (quad_add /
+// quad_dbl): for the narrow levels, where the operations are few and each
+// one's latency, not the throughput, sets the time.
+__global__ void __launch_bounds__(THREADS)
+    k_reduce_level_quad(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, uint32_t n_windows,
+                        bool first, u32* __restrict__ Xo, u32* __restrict__ Yo) {
+  const uint32_t Mo = M >> 1;
+  const uint32_t items = Mo * n_windows;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t op = gid >> 2, q = gid & 3;
+  if (op >= 2 * items) return;  // whole quads: 2 * items * 4 threads are launched in multiples of 4
+  const bool yside = op >= items;
+  const uint32_t it = yside ? op - items : op;
+  const size_t j0 = (size_t)(it / Mo) * M + 2 * (it % Mo);
+  const G1XYZZ x1 = G1XYZZ::load(X + (j0 + 1) * 32);
+  G1XYZZ r;
+  if (!yside) {
+    r = quad_dbl(quad_add(G1XYZZ::load(X + j0 * 32), x1, q), q);
+  } else {
+    const u32* src = first ? X : Y;
+    const G1XYZZ y1 = first ? x1 : G1XYZZ::load(Y + (j0 + 1) * 32);
+    r = quad_add(quad_add(G1XYZZ::load(src + j0 * 32), y1, q), x1, q);
+  }
+  if (q == 0) r.store((yside ? Yo : Xo) + (size_t)it * 32);
+}
+
+// k_reduce_tail with a quad of lanes per point operation: QUAD_TAIL_THREADS
+// threads per window, 4 per operation, X' operations below half of them.
+constexpr uint32_t QUAD_TAIL_THREADS = 4 * RED_TAIL;
+__global__ void __launch_bounds__(QUAD_TAIL_THREADS)
+    k_reduce_tail_quad(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, bool first,
+                       u32* __restrict__ out) {
+  __shared__ G1XYZZ sx[RED_TAIL], sy[RED_TAIL];
+  const size_t base = (size_t)blockIdx.x * M;
+  for (uint32_t j = threadIdx.x; j < M; j += QUAD_TAIL_THREADS) {
+    sx[j] = G1XYZZ::load(X + (base + j) * 32);
+    sy[j] = first ? sx[j] : G1XYZZ::load(Y + (base + j) * 32);
+  }
+  __syncthreads();
+  const uint32_t op = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const bool yside = op >= RED_TAIL / 2;
+  const uint32_t i = yside ? op - RED_TAIL / 2 : op;
+  for (uint32_t P = M >> 1; P >= 1; P >>= 1) {
+    const bool act = i < P;  // the same for the four lanes of a quad
+    G1XYZZ r;
+    if (act) {
+      const G1XYZZ x1 = sx[2 * i + 1];
+      r = yside ? quad_add(quad_add(sy[2 * i], sy[2 * i + 1], q), x1, q) : quad_dbl(quad_add(sx[2 * i], x1, q), q);
+    }
+    __syncthreads();
+    if (act && q == 0) {
+      if (yside)
+        sy[i] = r;
+      else
+        sx[i] = r;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sy[0].store(out + (size_t)blockIdx.x * 32);
+}
+
 // ---------------------------------------------------------------- precompute
 
 // table j (1 <= j < n_precompute) of point i = 2^(j * shift) * P_i, affine
@@ -1103,6 +1164,10 @@ static hipError_t accumulate_part(const Config& cfg, const Layout& L, char* ws, 
   return accumulate_sorted(cfg, L, ws, set, d_points, fresh, stream, pt);
 }
 
+// Levels with at most this many point operations run a quad of lanes per
+// operation (their time is the operations' latency, not their count).
+constexpr uint64_t QUAD_LEVEL_OPS = 65536;
+
 // Reduction of bucket set `set`; the n_windows window sums end at *out_dev (XYZZ).
 static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, uint32_t set, hipStream_t stream,
                                  const u32** out_dev, PhaseTimer& pt) {
@@ -1114,15 +1179,21 @@ static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, u
     u32* Xo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2]);
     u32* Yo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2 + 1]);
     const uint32_t Mo = L.red_m[rl];
-    hipLaunchKernelGGL(k_reduce_level, dim3(div_ceil32(2ull * Mo * cfg.n_windows, THREADS)), dim3(THREADS), 0,
-                       stream, X, Y, M, cfg.n_windows, rl == 0, Xo, Yo);
+    const uint64_t ops = 2ull * Mo * cfg.n_windows;
+    if (ops <= QUAD_LEVEL_OPS)
+      hipLaunchKernelGGL(k_reduce_level_quad, dim3(div_ceil32(4 * ops, THREADS)), dim3(THREADS), 0, stream, X, Y, M,
+                         cfg.n_windows, rl == 0, Xo, Yo);
+    else
+      hipLaunchKernelGGL(k_reduce_level, dim3(div_ceil32(ops, THREADS)), dim3(THREADS), 0, stream, X, Y, M,
+                         cfg.n_windows, rl == 0, Xo, Yo);
     ZKP_CHECK(hipGetLastError());
     X = Xo;
     Y = Yo;
     M = Mo;
   }
   u32* sums = reinterpret_cast<u32*>(ws + L.sums);
-  hipLaunchKernelGGL(k_reduce_tail, dim3(cfg.n_windows), dim3(THREADS), 0, stream, X, Y, M, L.red_levels == 0, sums);
+  hipLaunchKernelGGL(k_reduce_tail_quad, dim3(cfg.n_windows), dim3(QUAD_TAIL_THREADS), 0, stream, X, Y, M,
+                     L.red_levels == 0, sums);
   ZKP_CHECK(hipGetLastError());
   *out_dev = sums;
   pt.mark("reduce");
