@@ -751,8 +751,11 @@ struct Emitter {
       emit_scalar_outputs(o, "mut_vars", "  ", "");
       o << "}\n\n";
     }
+    // the occupancy the register budget was chosen for binds only kernels that spill; a small kernel keeps
+    // whatever occupancy its few registers give (its loads need the waves)
     const std::string bounds = "__launch_bounds__(zkp::fused::THREADS)" +
-        (waves_per_eu ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_per_eu) + ")))" : std::string());
+        (waves_per_eu && spills ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_per_eu) + ")))"
+                                : std::string());
     o << "__global__ void " << bounds << "\n    " << name
       << "_kernel(const ConstPolyPtr* __restrict__ vars, const PolyPtr* __restrict__ mut_vars, "
          "unsigned long long len, bool is_first, uint32_t* __restrict__ local) {\n";
