@@ -47,7 +47,7 @@ namespace msm {
 constexpr int THREADS = 256;
 constexpr uint32_t CHUNK = 32;   // sorted entries per thread at accumulation level 0
 constexpr uint32_t CHUNK_L = 8;  // (key, partial) entries per thread at the later levels
-constexpr uint32_t RED_TAIL = 256;  // items per window finished inside one workgroup
+constexpr uint32_t RED_TAIL = 128;  // items per window finished inside one workgroup
 constexpr uint32_t SCALAR_BITS = 254;
 constexpr int MAX_LEVELS = 24;
 
@@ -198,20 +198,26 @@ __global__ void __launch_bounds__(THREADS)
 constexpr int SCAN_BLOCK = 1024;  // elements per block: 256 threads x 4
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
-  // Hillis-Steele over 256 threads in LDS
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < THREADS; off <<= 1) {
-    uint32_t add = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
+  // inclusive scan inside each wave by shuffles, then the waves' totals through LDS
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
   }
-  const uint32_t incl = sh[t];
-  *total = sh[THREADS - 1];
+  if (lane == 63) sh[wave] = x;
   __syncthreads();
-  return incl - v;
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < THREADS / 64; w++) {
+    const uint32_t t = sh[w];
+    before += w < wave ? t : 0u;
+    all += t;
+  }
+  *total = all;
+  __syncthreads();  // sh is free again for the caller
+  return before + x - v;
 }
 
 __global__ void __launch_bounds__(THREADS) k_scan_blocks(uint32_t* data, uint32_t n, uint32_t* block_sums) {
@@ -377,7 +383,8 @@ __global__ void __launch_bounds__(THREADS)
 // window (s = 12, 13, 14, 18 and the small windows) is spread over many
 // workgroups.  Bucket counts of a one-slice segment are stored, those of a
 // longer one added up with one device atomic per nonzero bucket per slice.
-constexpr uint32_t SEG_SLICE = 16384;
+constexpr uint32_t SEG_SLICE = 8192;  // entries per workgroup, staged in LDS at once (64 KB)
+constexpr uint32_t SEG_PER_THREAD = SEG_SLICE / THREADS;
 
 // largest k < n with prefix[k] <= g (prefix: n + 1 non-decreasing entries)
 __device__ __forceinline__ uint32_t find_segment(const uint32_t* prefix, uint32_t n, uint32_t g) {
@@ -471,13 +478,12 @@ __global__ void __launch_bounds__(THREADS)
 }
 
 // Every slice takes its run of each bucket from the cursor (one atomic per
-// nonzero bucket; a one-slice segment owns the whole bucket), then moves its
-// entries in rounds of SEG_ROUND: a round is ranked by bucket in LDS, staged in
-// bucket order and written out so that each bucket's entries of the round land
-// as one contiguous run (coalesced stores instead of one scattered store per
-// entry).  The order inside a bucket is not kept.
-constexpr uint32_t SEG_ROUND = 8192;  // entries staged in LDS per round (64 KB)
-
+// nonzero bucket; a one-slice segment owns the whole bucket) and moves its
+// entries in one round: each thread holds its entries in registers with their
+// rank inside their bucket (the value the counting atomic returned), stages
+// them in bucket order in LDS, and the slice is written out so that each
+// bucket's entries land as one contiguous run (coalesced stores instead of one
+// scattered store per entry).  The order inside a bucket is not kept.
 __global__ void __launch_bounds__(THREADS)
     k_seg_scatter(const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ slice_prefix, SortPlan sp,
@@ -491,49 +497,50 @@ __global__ void __launch_bounds__(THREADS)
   uint32_t* lbase = h + nb;
   uint32_t* gcur = h + 2 * nb;
   uint32_t* skey = h + 3 * nb;
-  uint32_t* sval = skey + SEG_ROUND;
+  uint32_t* sval = skey + SEG_SLICE;
   for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = 0;
   __syncthreads();
-  for (uint32_t e = sl.lo + threadIdx.x; e < sl.hi; e += THREADS) atomicAdd(&cnt[keys_in[e] & mask], 1u);
+  uint32_t key[SEG_PER_THREAD], val[SEG_PER_THREAD], rank[SEG_PER_THREAD];
+#pragma unroll
+  for (uint32_t k = 0; k < SEG_PER_THREAD; k++) {
+    const uint32_t e = sl.lo + k * THREADS + threadIdx.x;
+    if (e < sl.hi) {
+      key[k] = keys_in[e];
+      val[k] = vals_in[e];
+      rank[k] = atomicAdd(&cnt[key[k] & mask], 1u);
+    }
+  }
   __syncthreads();
   uint32_t* cur = cursor + (uint64_t)sl.cb * nb;
   for (uint32_t b = threadIdx.x; b < nb; b += THREADS) {
     const uint32_t c = cnt[b];
     gcur[b] = sl.single ? cur[b] : (c ? atomicAdd(&cur[b], c) : 0u);
+    lbase[b] = c;
   }
-  for (uint32_t r0 = sl.lo; r0 < sl.hi; r0 += SEG_ROUND) {
-    const uint32_t r1 = r0 + SEG_ROUND < sl.hi ? r0 + SEG_ROUND : sl.hi;
-    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = 0;
-    __syncthreads();
-    for (uint32_t e = r0 + threadIdx.x; e < r1; e += THREADS) atomicAdd(&cnt[keys_in[e] & mask], 1u);
-    __syncthreads();
-    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) lbase[b] = cnt[b];
-    __syncthreads();
-    lds_exclusive_scan(lbase, nb, sh);
-    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) cnt[b] = lbase[b];  // staging cursor
-    __syncthreads();
-    for (uint32_t e = r0 + threadIdx.x; e < r1; e += THREADS) {
-      const uint32_t key = keys_in[e];
-      const uint32_t slot = atomicAdd(&cnt[key & mask], 1u);
-      skey[slot] = key;
-      sval[slot] = vals_in[e];
+  __syncthreads();
+  lds_exclusive_scan(lbase, nb, sh);
+#pragma unroll
+  for (uint32_t k = 0; k < SEG_PER_THREAD; k++) {
+    const uint32_t e = sl.lo + k * THREADS + threadIdx.x;
+    if (e < sl.hi) {
+      const uint32_t slot = lbase[key[k] & mask] + rank[k];
+      skey[slot] = key[k];
+      sval[slot] = val[k];
     }
-    __syncthreads();
-    for (uint32_t slot = threadIdx.x; slot < r1 - r0; slot += THREADS) {
-      const uint32_t key = skey[slot];
-      const uint32_t b = key & mask;
-      const uint32_t dst = gcur[b] + (slot - lbase[b]);
-      keys_out[dst] = key;
-      vals_out[dst] = sval[slot];
-    }
-    __syncthreads();
-    for (uint32_t b = threadIdx.x; b < nb; b += THREADS) gcur[b] += cnt[b] - lbase[b];  // this round's entries
-    __syncthreads();
+  }
+  __syncthreads();
+  const uint32_t n = sl.hi - sl.lo;
+  for (uint32_t slot = threadIdx.x; slot < n; slot += THREADS) {
+    const uint32_t k = skey[slot];
+    const uint32_t b = k & mask;
+    const uint32_t dst = gcur[b] + (slot - lbase[b]);
+    keys_out[dst] = k;
+    vals_out[dst] = sval[slot];
   }
 }
 
 // Dynamic LDS of k_seg_scatter for 2^low_bits buckets per segment.
-static inline uint32_t seg_scatter_lds(uint32_t low_bits) { return (3u * (1u << low_bits) + 2u * SEG_ROUND) * 4u; }
+static inline uint32_t seg_scatter_lds(uint32_t low_bits) { return (3u * (1u << low_bits) + 2u * SEG_SLICE) * 4u; }
 
 // ---------------------------------------------------------------- accumulation
 
@@ -727,40 +734,6 @@ __global__ void __launch_bounds__(THREADS)
   }
 }
 
-// The last levels of window blockIdx.x (M <= RED_TAIL items) in LDS: threads
-// below THREADS / 2 compute X', the others Y'; out[w] = S_w.
-__global__ void __launch_bounds__(THREADS)
-    k_reduce_tail(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, bool first,
-                  u32* __restrict__ out) {
-  __shared__ G1XYZZ sx[RED_TAIL], sy[RED_TAIL];
-  static_assert(RED_TAIL / 2 <= THREADS / 2, "one pair per thread and side");
-  const size_t base = (size_t)blockIdx.x * M;
-  for (uint32_t j = threadIdx.x; j < M; j += THREADS) {
-    sx[j] = G1XYZZ::load(X + (base + j) * 32);
-    sy[j] = first ? sx[j] : G1XYZZ::load(Y + (base + j) * 32);
-  }
-  __syncthreads();
-  const bool yside = threadIdx.x >= THREADS / 2;
-  const uint32_t i = yside ? threadIdx.x - THREADS / 2 : threadIdx.x;
-  for (uint32_t P = M >> 1; P >= 1; P >>= 1) {
-    const bool act = i < P;
-    G1XYZZ r;
-    if (act) {
-      const G1XYZZ x1 = sx[2 * i + 1];
-      r = yside ? sy[2 * i].add(sy[2 * i + 1]).add(x1) : sx[2 * i].add(x1).dbl();
-    }
-    __syncthreads();
-    if (act) {
-      if (yside)
-        sy[i] = r;
-      else
-        sx[i] = r;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sy[0].store(out + (size_t)blockIdx.x * 32);
-}
-
 // The same level with a quad of lanes per point operation (quad_add /
 // quad_dbl): for the narrow levels, where the operations are few and each
 // one's latency, not the throughput, sets the time.
@@ -787,8 +760,11 @@ __global__ void __launch_bounds__(THREADS)
   if (q == 0) r.store((yside ? Yo : Xo) + (size_t)it * 32);
 }
 
-// k_reduce_tail with a quad of lanes per point operation: QUAD_TAIL_THREADS
-// threads per window, 4 per operation, X' operations below half of them.
+// The last levels of window blockIdx.x (M <= RED_TAIL items) in LDS, all in
+// one launch, a quad of lanes per point operation: QUAD_TAIL_THREADS threads,
+// the operations writing X' on the lower half of them; out[w] = S_w.  At
+// 512 threads the kernel may keep 256 VGPRs, which the quad's operand copies
+// need (at 1024 it spilled).
 constexpr uint32_t QUAD_TAIL_THREADS = 4 * RED_TAIL;
 __global__ void __launch_bounds__(QUAD_TAIL_THREADS)
     k_reduce_tail_quad(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, bool first,
