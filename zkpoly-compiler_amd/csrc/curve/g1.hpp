@@ -169,102 +169,4 @@ struct G1XYZZ {
   }
 };
 
-// ---- quad-cooperative XYZZ arithmetic (device) ----
-//
-// Where few point operations are left to do (the last levels of the bucket
-// reduction) each one runs on a single lane and its 14 products come one
-// after another.  Here the four lanes of a quad (q = lane & 3) hold the same
-// operands and split every phase of independent products among themselves:
-// lane q computes product q, and DPP quad permutes hand every product to all
-// four lanes.  add-2008-s has four such phases (4, 4, 3, 3 products) and
-// dbl-2008-s-1 three (2, 4, 3), so an operation takes four or three product
-// latencies instead of 14 or 9.  Every branch depends only on the shared
-// operands, so the four lanes of a quad always run together.
-template <int SRC>
-__device__ __forceinline__ Fq quad_from(const Fq& x) {
-  Fq r;
-#pragma unroll
-  for (int i = 0; i < 8; i++) r.v[i] = (u32)__builtin_amdgcn_mov_dpp((int)x.v[i], SRC * 0x55, 0xf, 0xf, false);
-  return r;
-}
-
-// lane q of the quad takes operand q.  The choice is a bit-select on masks of
-// q's two bits (v_bfi_b32), kept opaque: written as a select over the operands
-// the optimiser folds it into a load of operand[q] from a stack copy.
-__device__ __forceinline__ u32 quad_bfi(u32 m, u32 x1, u32 x0) {
-  u32 r;
-  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(x1), "v"(x0));
-  return r;
-}
-
-__device__ __forceinline__ Fq quad_pick(u32 q, const Fq& a, const Fq& b, const Fq& c, const Fq& d) {
-  const u32 m1 = 0u - (q & 1u), m2 = 0u - ((q >> 1) & 1u);
-  Fq r;
-#pragma unroll
-  for (int i = 0; i < 8; i++) r.v[i] = quad_bfi(m2, quad_bfi(m1, d.v[i], c.v[i]), quad_bfi(m1, b.v[i], a.v[i]));
-  return r;
-}
-
-// out[k] = l[k] * r[k] for k < N (N <= 4) on every lane of the quad
-template <int N>
-__device__ __forceinline__ void quad_mul(u32 q, const Fq (&l)[N], const Fq (&r)[N], Fq (&out)[N]) {
-  const Fq a = quad_pick(q, l[0], l[N > 1 ? 1 : 0], l[N > 2 ? 2 : 0], l[N > 3 ? 3 : 0]);
-  const Fq b = quad_pick(q, r[0], r[N > 1 ? 1 : 0], r[N > 2 ? 2 : 0], r[N > 3 ? 3 : 0]);
-  const Fq m = a * b;
-  out[0] = quad_from<0>(m);
-  if (N > 1) out[N > 1 ? 1 : 0] = quad_from<1>(m);
-  if (N > 2) out[N > 2 ? 2 : 0] = quad_from<2>(m);
-  if (N > 3) out[N > 3 ? 3 : 0] = quad_from<3>(m);
-}
-
-// 2a on the quad (G1XYZZ::dbl's formula)
-__device__ __forceinline__ G1XYZZ quad_dbl(const G1XYZZ& a, u32 q) {
-  if (a.is_identity()) return a;
-  const Fq u = a.y.dbl();
-  Fq p1[2];
-  quad_mul<2>(q, {u, a.x}, {u, a.x}, p1);  // V, X^2
-  const Fq v = p1[0];
-  const Fq m = p1[1].dbl() + p1[1];
-  Fq p2[4];
-  quad_mul<4>(q, {u, a.x, m, v}, {v, v, m, a.zz}, p2);  // W, S, M^2, ZZ3
-  const Fq w = p2[0], s = p2[1];
-  G1XYZZ r;
-  r.x = p2[2] - s.dbl();
-  r.zz = p2[3];
-  Fq p3[3];
-  quad_mul<3>(q, {m, w, w}, {s - r.x, a.y, a.zzz}, p3);
-  r.y = p3[0] - p3[1];
-  r.zzz = p3[2];
-  return r;
-}
-
-// a + b on the quad (G1XYZZ::add's formula, equal and opposite points included)
-__device__ __forceinline__ G1XYZZ quad_add(const G1XYZZ& a, const G1XYZZ& o, u32 q) {
-  if (o.is_identity()) return a;
-  if (a.is_identity()) return o;
-  Fq p1[4];
-  quad_mul<4>(q, {a.x, o.x, a.y, o.y}, {o.zz, a.zz, o.zzz, a.zzz}, p1);  // U1, U2, S1, S2
-  const Fq u1 = p1[0], s1 = p1[2];
-  const Fq p = p1[1] - u1;
-  const Fq rr = p1[3] - s1;
-  if (p.is_zero()) {
-    if (rr.is_zero()) return quad_dbl(a, q);
-    return G1XYZZ::identity();
-  }
-  Fq p2[4];
-  quad_mul<4>(q, {p, rr, a.zz, a.zzz}, {p, rr, o.zz, o.zzz}, p2);  // PP, R^2, ZZ1 ZZ2, ZZZ1 ZZZ2
-  const Fq pp = p2[0];
-  Fq p3[3];
-  quad_mul<3>(q, {p, u1, p2[2]}, {pp, pp, pp}, p3);  // PPP, Q, ZZ3
-  const Fq ppp = p3[0], qq = p3[1];
-  G1XYZZ r;
-  r.x = p2[1] - ppp - qq.dbl();
-  r.zz = p3[2];
-  Fq p4[3];
-  quad_mul<3>(q, {rr, s1, p2[3]}, {qq - r.x, ppp, ppp}, p4);
-  r.y = p4[0] - p4[1];
-  r.zzz = p4[2];
-  return r;
-}
-
 }  // namespace zkp

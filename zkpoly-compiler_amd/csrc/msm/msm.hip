@@ -21,11 +21,14 @@
 //     additions); runs that cross a chunk edge become (key, partial) pairs
 //     that the next level reduces the same way, so a bucket holding any share
 //     of the points (e.g. the scalar-1 bucket) costs log levels, not a serial walk.
+//     Buckets and partials are G1X29 (xyzz29.hpp): Fq in nine 29-bit limbs,
+//     radix 2^261, lazily reduced -- one instruction per limb product.
 //  4. bucket reduction per window, S_w = sum_b b * B_b, by halving levels
 //     (X, Y) -> (2 (X_2i + X_2i+1), X_2i+1 + Y_2i + Y_2i+1): X' and Y' run on
 //     different threads, two point operations deep per level, since a chain of
-//     point additions is latency bound; the last 256 items of a window are
-//     halved inside one workgroup (LDS), all levels in one launch.
+//     point additions is latency bound; the last 128 items of a window are
+//     halved inside one workgroup (LDS), all levels in one launch, and the
+//     window sums leave in G1XYZZ's form (8 x 32-bit limbs, radix 2^256).
 //  5. host: Horner over the window sums (s doublings each) and one inversion
 //     to affine -- the serial tail runs where a modular product is ~30x lower
 //     latency than one GPU wave (measured 738 ns per dependent product).
@@ -40,6 +43,7 @@
 #include <vector>
 
 #include "../curve/g1.hpp"
+#include "../curve/xyzz29.hpp"
 
 namespace zkp {
 namespace msm {
@@ -548,21 +552,19 @@ static inline uint32_t seg_scatter_lds(uint32_t low_bits) { return (3u * (1u << 
 // run, or all its partials, inside one chunk), so in the first part of an MSM
 // (fresh buckets hold the identity) the sum is stored, in later parts added.
 // Identity sums are skipped: slots without a partial carry a key too.
-__device__ __forceinline__ void bucket_close(u32* buckets, uint32_t key, const G1XYZZ& sum, bool fresh) {
+__device__ __forceinline__ void bucket_close(u32* buckets, uint32_t key, const G1X29& sum, bool fresh) {
   if (sum.is_identity()) return;
   u32* p = buckets + (size_t)key * 32;
   if (fresh) {
     sum.store(p);
     return;
   }
-  const G1XYZZ cur = G1XYZZ::load(p);
+  const G1X29 cur = G1X29::load(p);
   (cur.is_identity() ? sum : cur.add(sum)).store(p);
 }
 
-__device__ __forceinline__ G1Affine fetch_point(const u32* __restrict__ points, uint32_t v) {
-  G1Affine a = G1Affine::load(points + (size_t)(v & 0x7fffffffu) * 16);
-  if (v & 0x80000000u) a = a.neg();
-  return a;
+__device__ __forceinline__ Affine29 fetch_point(const u32* __restrict__ points, uint32_t v) {
+  return Affine29::load(points + (size_t)(v & 0x7fffffffu) * 16, (v & 0x80000000u) != 0);
 }
 
 // One accumulation level.  Thread t owns sorted entries [t*CH, ...) and sums
@@ -588,14 +590,14 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
   const bool open_left = start > 0 && keys[start - 1] == key_first;
   const bool open_right = end < N && keys[end] == key_last;
   bool a_done = false, b_done = false;
-  G1XYZZ acc = G1XYZZ::identity();
+  G1X29 acc = G1X29::identity();
   uint32_t run_start = start;
   for (uint32_t i = start; i < end; i++) {
     const uint32_t key = keys[i];
     if (FROM_POINTS)
       acc = acc.add_affine(fetch_point(src, vals[i]));
     else
-      acc = acc.add(G1XYZZ::load(src + (size_t)i * 32));
+      acc = acc.add(G1X29::load(src + (size_t)i * 32));
     const bool run_end = (i + 1 == end) || keys[i + 1] != key;
     if (run_end) {
       const bool first = run_start == start, last = i + 1 == end;
@@ -608,20 +610,22 @@ __device__ __forceinline__ void accumulate_chunk(const uint32_t* __restrict__ ke
       } else {
         bucket_close(buckets, key, acc, fresh);
       }
-      acc = G1XYZZ::identity();
+      acc = G1X29::identity();
       run_start = i + 1;
     }
   }
   if (next_keys) {
     next_keys[2 * t] = key_first;
     next_keys[2 * t + 1] = key_last;
-    if (!a_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t) * 32);
-    if (!b_done) G1XYZZ::identity().store(next_pts + (size_t)(2 * t + 1) * 32);
+    if (!a_done) G1X29::identity().store(next_pts + (size_t)(2 * t) * 32);
+    if (!b_done) G1X29::identity().store(next_pts + (size_t)(2 * t + 1) * 32);
   }
 }
 
-// Level 0: entries are (key, point index | sign << 31).
-__global__ void __launch_bounds__(THREADS)
+// Level 0: entries are (key, point index | sign << 31).  Three waves per SIMD
+// (168 VGPRs, a few values spilled) run it faster than two without spills: the
+// dependent loads of an entry (key, point index, point) wait less.
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     k_accumulate0(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                   const u32* __restrict__ points, uint32_t* counts, bool fresh, u32* buckets,
                   uint32_t* __restrict__ next_keys, u32* __restrict__ next_pts) {
@@ -662,10 +666,10 @@ __global__ void __launch_bounds__(THREADS)
     *run_flag = 1u;
     return;
   }
-  G1XYZZ acc = G1XYZZ::identity();
+  G1X29 acc = G1X29::identity();
   for (uint32_t i = s; i < e; i++) {
     u32* p = pts + (size_t)i * 32;
-    acc = acc.add(G1XYZZ::load(p));
+    acc = acc.add(G1X29::load(p));
     Fq::zero().store(p + 16);
   }
   bucket_close(buckets, key, acc, fresh);
@@ -691,10 +695,10 @@ __global__ void __launch_bounds__(THREADS) k_merge_buckets(u32* __restrict__ dst
                                                            uint32_t H) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= H) return;
-  const G1XYZZ b = G1XYZZ::load(src + (size_t)i * 32);
+  const G1X29 b = G1X29::load(src + (size_t)i * 32);
   if (b.is_identity()) return;
   u32* d = dst + (size_t)i * 32;
-  const G1XYZZ a = G1XYZZ::load(d);
+  const G1X29 a = G1X29::load(d);
   (a.is_identity() ? b : a.add(b)).store(d);
 }
 
@@ -724,18 +728,18 @@ __global__ void __launch_bounds__(THREADS)
   const bool yside = gid >= items;
   const uint32_t it = yside ? gid - items : gid;
   const size_t j0 = (size_t)(it / Mo) * M + 2 * (it % Mo);
-  const G1XYZZ x1 = G1XYZZ::load(X + (j0 + 1) * 32);
+  const G1X29 x1 = G1X29::load(X + (j0 + 1) * 32);
   if (!yside) {
-    G1XYZZ::load(X + j0 * 32).add(x1).dbl().store(Xo + (size_t)it * 32);
+    G1X29::load(X + j0 * 32).add(x1).dbl().store(Xo + (size_t)it * 32);
   } else {
     const u32* src = first ? X : Y;
-    const G1XYZZ y1 = first ? x1 : G1XYZZ::load(Y + (j0 + 1) * 32);
-    G1XYZZ::load(src + j0 * 32).add(y1).add(x1).store(Yo + (size_t)it * 32);
+    const G1X29 y1 = first ? x1 : G1X29::load(Y + (j0 + 1) * 32);
+    G1X29::load(src + j0 * 32).add(y1).add(x1).store(Yo + (size_t)it * 32);
   }
 }
 
-// The same level with a quad of lanes per point operation (quad_add /
-// quad_dbl): for the narrow levels, where the operations are few and each
+// The same level with a quad of lanes per point operation (quad29_add /
+// quad29_dbl): for the narrow levels, where the operations are few and each
 // one's latency, not the throughput, sets the time.
 __global__ void __launch_bounds__(THREADS)
     k_reduce_level_quad(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, uint32_t n_windows,
@@ -748,14 +752,14 @@ __global__ void __launch_bounds__(THREADS)
   const bool yside = op >= items;
   const uint32_t it = yside ? op - items : op;
   const size_t j0 = (size_t)(it / Mo) * M + 2 * (it % Mo);
-  const G1XYZZ x1 = G1XYZZ::load(X + (j0 + 1) * 32);
-  G1XYZZ r;
+  const G1X29 x1 = G1X29::load(X + (j0 + 1) * 32);
+  G1X29 r;
   if (!yside) {
-    r = quad_dbl(quad_add(G1XYZZ::load(X + j0 * 32), x1, q), q);
+    r = quad29_dbl(quad29_add(G1X29::load(X + j0 * 32), x1, q), q);
   } else {
     const u32* src = first ? X : Y;
-    const G1XYZZ y1 = first ? x1 : G1XYZZ::load(Y + (j0 + 1) * 32);
-    r = quad_add(quad_add(G1XYZZ::load(src + j0 * 32), y1, q), x1, q);
+    const G1X29 y1 = first ? x1 : G1X29::load(Y + (j0 + 1) * 32);
+    r = quad29_add(quad29_add(G1X29::load(src + j0 * 32), y1, q), x1, q);
   }
   if (q == 0) r.store((yside ? Yo : Xo) + (size_t)it * 32);
 }
@@ -769,11 +773,11 @@ constexpr uint32_t QUAD_TAIL_THREADS = 4 * RED_TAIL;
 __global__ void __launch_bounds__(QUAD_TAIL_THREADS)
     k_reduce_tail_quad(const u32* __restrict__ X, const u32* __restrict__ Y, uint32_t M, bool first,
                        u32* __restrict__ out) {
-  __shared__ G1XYZZ sx[RED_TAIL], sy[RED_TAIL];
+  __shared__ G1X29 sx[RED_TAIL], sy[RED_TAIL];
   const size_t base = (size_t)blockIdx.x * M;
   for (uint32_t j = threadIdx.x; j < M; j += QUAD_TAIL_THREADS) {
-    sx[j] = G1XYZZ::load(X + (base + j) * 32);
-    sy[j] = first ? sx[j] : G1XYZZ::load(Y + (base + j) * 32);
+    sx[j] = G1X29::load(X + (base + j) * 32);
+    sy[j] = first ? sx[j] : G1X29::load(Y + (base + j) * 32);
   }
   __syncthreads();
   const uint32_t op = threadIdx.x >> 2, q = threadIdx.x & 3;
@@ -781,10 +785,11 @@ __global__ void __launch_bounds__(QUAD_TAIL_THREADS)
   const uint32_t i = yside ? op - RED_TAIL / 2 : op;
   for (uint32_t P = M >> 1; P >= 1; P >>= 1) {
     const bool act = i < P;  // the same for the four lanes of a quad
-    G1XYZZ r;
+    G1X29 r;
     if (act) {
-      const G1XYZZ x1 = sx[2 * i + 1];
-      r = yside ? quad_add(quad_add(sy[2 * i], sy[2 * i + 1], q), x1, q) : quad_dbl(quad_add(sx[2 * i], x1, q), q);
+      const G1X29 x1 = sx[2 * i + 1];
+      r = yside ? quad29_add(quad29_add(sy[2 * i], sy[2 * i + 1], q), x1, q)
+                : quad29_dbl(quad29_add(sx[2 * i], x1, q), q);
     }
     __syncthreads();
     if (act && q == 0) {
@@ -795,7 +800,7 @@ __global__ void __launch_bounds__(QUAD_TAIL_THREADS)
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) sy[0].store(out + (size_t)blockIdx.x * 32);
+  if (threadIdx.x == 0) sy[0].store_mont256(out + (size_t)blockIdx.x * 32);  // the host reads G1XYZZ
 }
 
 // ---------------------------------------------------------------- precompute
