@@ -339,40 +339,90 @@ __device__ __forceinline__ void lds_exclusive_scan(uint32_t* a, uint32_t m, uint
   __syncthreads();
 }
 
+// Points per thread and windows per point whose entries k_part_scatter keeps
+// in registers, with each entry's rank in its bin from the counting atomic:
+// one digit pass over the tile instead of a counting pass and a placing pass.
+// Other tile shapes take the two passes.
+constexpr uint32_t PART_PPT = 2, PART_WMAX = 22;
+
 __global__ void __launch_bounds__(THREADS)
     k_part_scatter(const uint32_t* __restrict__ scalars, uint64_t n, Config cfg, SortPlan sp,
                    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
                    uint32_t* __restrict__ vals_out) {
   extern __shared__ uint32_t dyn[];
   __shared__ uint32_t sh[THREADS];
-  uint32_t* base = dyn;               // nc: first staging slot of each bin
-  uint32_t* cur = dyn + sp.nc;        // nc: staging cursor
+  uint32_t* base = dyn;               // nc: entries of each bin, then its first staging slot
+  uint32_t* cur = dyn + sp.nc;        // nc: entries of each bin / staging cursor
   uint32_t* skey = dyn + 2 * sp.nc;   // tp * actual_windows
   uint32_t* sval = skey + (size_t)sp.tp * cfg.actual_windows;
   const uint64_t first = (uint64_t)blockIdx.x * sp.tp;
   for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) base[b] = 0;
   __syncthreads();
-  part_count_tile(scalars, n, cfg, sp.low_bits, first, sp.tp, base);
-  __syncthreads();
-  lds_exclusive_scan(base, sp.nc, sh);
-  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) cur[b] = base[b];
-  __syncthreads();
-  for (uint32_t p = threadIdx.x; p < sp.tp; p += THREADS) {
-    const uint64_t i = first + p;
-    if (i >= n) break;
-    const Fr c = Fr::load(scalars + i * 8).to_canonical();
-    int carry = 0;
-    for (uint32_t w = 0; w < cfg.actual_windows; w++) {
-      const int d = next_digit(c, w, cfg.s, carry);
-      if (d == 0) continue;
-      const uint32_t key = (w % cfg.n_windows) * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
-      const uint32_t slot = atomicAdd(&cur[key >> sp.low_bits], 1u);
-      skey[slot] = key;
-      sval[slot] = (uint32_t)((w / cfg.n_windows) * n + i) | (d < 0 ? 0x80000000u : 0u);
+  const bool regs = sp.tp == PART_PPT * THREADS && cfg.actual_windows <= PART_WMAX;  // uniform
+  uint32_t ek[PART_PPT][PART_WMAX], er[PART_PPT][PART_WMAX];  // key | sign << 31 (~0: none), rank
+  if (regs) {
+#pragma unroll
+    for (uint32_t p = 0; p < PART_PPT; p++) {
+      const uint64_t i = first + p * THREADS + threadIdx.x;
+      const bool valid = i < n;
+      const Fr c = valid ? Fr::load(scalars + i * 8).to_canonical() : Fr::zero();
+      int carry = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < PART_WMAX; w++) {
+        ek[p][w] = ~0u;
+        er[p][w] = 0;
+        if (w < cfg.actual_windows) {
+          const int d = next_digit(c, w, cfg.s, carry);
+          if (valid && d != 0) {
+            const uint32_t key = (w % cfg.n_windows) * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+            ek[p][w] = key | (d < 0 ? 0x80000000u : 0u);
+            er[p][w] = atomicAdd(&base[key >> sp.low_bits], 1u);
+          }
+        }
+      }
     }
+  } else {
+    part_count_tile(scalars, n, cfg, sp.low_bits, first, sp.tp, base);
   }
   __syncthreads();
-  const uint32_t staged = cur[sp.nc - 1];  // the last bin's cursor ends at the tile's entry count
+  for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) cur[b] = base[b];
+  lds_exclusive_scan(base, sp.nc, sh);
+  uint32_t staged;
+  if (regs) {
+#pragma unroll
+    for (uint32_t p = 0; p < PART_PPT; p++) {
+      const uint32_t i = (uint32_t)(first + p * THREADS + threadIdx.x);
+#pragma unroll
+      for (uint32_t w = 0; w < PART_WMAX; w++) {
+        if (ek[p][w] == ~0u) continue;
+        const uint32_t key = ek[p][w] & 0x7fffffffu;
+        const uint32_t slot = base[key >> sp.low_bits] + er[p][w];
+        skey[slot] = key;
+        sval[slot] = (uint32_t)((w / cfg.n_windows) * n + i) | (ek[p][w] & 0x80000000u);
+      }
+    }
+    staged = base[sp.nc - 1] + cur[sp.nc - 1];
+  } else {
+    for (uint32_t b = threadIdx.x; b < sp.nc; b += THREADS) cur[b] = base[b];
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < sp.tp; p += THREADS) {
+      const uint64_t i = first + p;
+      if (i >= n) break;
+      const Fr c = Fr::load(scalars + i * 8).to_canonical();
+      int carry = 0;
+      for (uint32_t w = 0; w < cfg.actual_windows; w++) {
+        const int d = next_digit(c, w, cfg.s, carry);
+        if (d == 0) continue;
+        const uint32_t key = (w % cfg.n_windows) * cfg.buckets + (uint32_t)(d < 0 ? -d : d) - 1;
+        const uint32_t slot = atomicAdd(&cur[key >> sp.low_bits], 1u);
+        skey[slot] = key;
+        sval[slot] = (uint32_t)((w / cfg.n_windows) * n + i) | (d < 0 ? 0x80000000u : 0u);
+      }
+    }
+    __syncthreads();
+    staged = cur[sp.nc - 1];  // the last bin's cursor ends at the tile's entry count
+  }
+  __syncthreads();
   for (uint32_t slot = threadIdx.x; slot < staged; slot += THREADS) {
     const uint32_t key = skey[slot];
     const uint32_t b = key >> sp.low_bits;
