@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--msm-only", action="store_true", help="MSM 2^20 leg only (for profiling)")
     ap.add_argument("--batched-only", action="store_true", help="config-4 batched leg only")
     ap.add_argument("--serving-only", action="store_true", help="config-5 serving leg only")
-    ap.add_argument("--serve-count", type=int, default=60, help="requests per rate in the serving leg")
+    ap.add_argument("--serve-count", type=int, default=200, help="requests per rate in the serving leg")
     ap.add_argument("--msm-log-n", type=int, default=20)
     ap.add_argument("--poly-only", action="store_true", help="libpoly.so leg only")
     ap.add_argument("--pipeline-only", action="store_true", help="pipelined fused-kernel leg only")
@@ -354,7 +354,19 @@ def msm_abi_leg(libs, pb, ss, n, steps, expected):
         "planned": (planned, 1),
         "w16_four_batches_one_run": (MsmConfig(16, 16, batch_per_run=4), 4),
     }
-    out = {}
+    # the host link's rate on this machine, from the same pinned points and the way the library copies
+    # (hipMemcpyAsync on a stream): every case moves 96 B per point
+    h = hip._lib()
+    link = hip.DeviceBuffer(n * 64)
+    cs = hip.Stream(0)
+    for k in range(5):
+        if k == 1:
+            t = time.perf_counter()
+        hip.check(h.hipMemcpyAsync(ctypes.c_void_p(link.addr), ctypes.c_void_p(pts), n * 64,
+                                   hip.hipMemcpyHostToDevice, cs.raw()), "link copy")
+        cs.sync()
+    out = {"link_h2d_gbs": round(4 * n * 64 / (time.perf_counter() - t) / 1e9, 2)}
+    del link
     for name, (cfg, batches) in cases.items():
         op = MSM(libs, cfg)
         size = op.get_buffer_size(n)[0]

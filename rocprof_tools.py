@@ -19,6 +19,19 @@ VALUUtilization (percent of active lanes) per kernel, from runs of
 the per-kernel table of a `rocprofv3 --kernel-trace --stats` run (DIR holds its
 run_kernel_stats.csv) as markdown, with the command that made it, and the
 average of KERNEL's last N dispatches (bench.py's timed region).
+
+`python3 rocprof_tools.py timeline DIR TITLE COMMAND OUT.md` reads a
+`rocprofv3 --kernel-trace --memory-copy-trace` run (DIR holds
+run_kernel_trace.csv and run_memory_copy_trace.csv), cuts it after every
+batch's window sums leave for the host, and tabulates the pieces that copy host data in and run
+kernels (the end-to-end msm symbol): their span, the time the copy engines
+and the kernels were busy, and how much of the copy time overlapped kernels;
+then lists the events of the last piece with the most parts.
+
+`python3 rocprof_tools.py msmcall DIR WINDOWS CALLS OUT.md` appends to OUT.md
+the average time per kernel of the first CALLS device-resident MSM calls
+with WINDOWS windows in a kernel trace (calls cut after the copy of their
+window sums; WINDOWS read from the reduction tail's grid).
 """
 from __future__ import annotations
 
@@ -171,6 +184,128 @@ def summarize_trace(trace_dir: str, title: str, command: str, out_md: str, tail_
         f.write("\n".join(lines) + "\n")
 
 
+def _union(iv):
+    """Total length of a union of [start, end) intervals, and the merged intervals."""
+    out = []
+    for a, b in sorted(iv):
+        if out and a <= out[-1][1]:
+            out[-1][1] = max(out[-1][1], b)
+        else:
+            out.append([a, b])
+    return sum(b - a for a, b in out), out
+
+
+def _overlap(xs, ys):
+    tot = 0
+    for a, b in xs:
+        for c, d in ys:
+            lo, hi = max(a, c), min(b, d)
+            if hi > lo:
+                tot += hi - lo
+    return tot
+
+
+def summarize_timeline(trace_dir: str, title: str, command: str, out_md: str) -> None:
+    ev = []
+    with open(f"{trace_dir}/run_kernel_trace.csv") as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0].split("::")[-1]
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), "kernel", name))
+    with open(f"{trace_dir}/run_memory_copy_trace.csv") as f:
+        for r in csv.DictReader(f):
+            kind = r["Direction"].replace("MEMORY_COPY_", "").lower()
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), "copy", kind))
+    ev.sort()
+    # a batch ends with the copy of its window sums to the host, after the
+    # reduction's tail (a copy engine transfer, or a blit kernel for so few bytes)
+    calls, cur, reduced = [], [], False
+    for e in ev:
+        cur.append(e)
+        if e[3].startswith("k_reduce_tail"):
+            reduced = True
+        elif reduced and (e[3] == "device_to_host" or "copyBuffer" in e[3]):
+            calls.append(cur)
+            cur, reduced = [], False
+    rows = []
+    for c in calls:
+        h2d = [(a, b) for a, b, k, n in c if k == "copy" and n == "host_to_device"]
+        ker = [(a, b) for a, b, k, n in c if k == "kernel"]
+        parts = sum(1 for e in c if e[3] == "k_part_count")
+        if not h2d or not parts:
+            continue
+        t0, t1 = min(e[0] for e in c), max(e[1] for e in c)
+        cbusy, cu = _union(h2d)
+        kbusy, ku = _union(ker)
+        rows.append((c, parts, len(h2d), (t1 - t0) / 1e6, cbusy / 1e6, kbusy / 1e6, _overlap(cu, ku) / 1e6))
+    lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
+             "## End-to-end batches (host inputs copied in, kernels, window sums back)", "",
+             "The trace is cut after every copy of a batch's window sums to the host; a batch that shares a "
+             "run with others carries the run's copies in its first piece.  `copy busy`: time at least one "
+             "host-to-device copy ran; `kernel busy`: time at least one kernel ran; `overlap`: copy time during "
+             "which a kernel also ran.", "",
+             "| piece | sorts (parts) | H2D copies | span ms | copy busy ms | kernel busy ms | overlap ms |",
+             "|---|---|---|---|---|---|---|"]
+    for i, (c, parts, ncopy, span, cb, kb, ov) in enumerate(rows):
+        lines.append(f"| {i} | {parts} | {ncopy} | {span:.3f} | {cb:.3f} | {kb:.3f} | {ov:.3f} |")
+    if rows:
+        best = max(rows, key=lambda r: (r[1], rows.index(r)))
+        c = best[0]
+        t0 = min(e[0] for e in c)
+        lines += ["", f"## Events of the last piece with {best[1]} sorts (us from its first event)", "",
+                  "| start | duration | kind | name |", "|---|---|---|---|"]
+        for a, b, k, n in c:
+            lines.append(f"| {(a - t0) / 1e3:.1f} | {(b - a) / 1e3:.1f} | {k} | `{n}` |")
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def summarize_msm_call(trace_dir: str, windows: int, calls: int, out_md: str) -> None:
+    ev = []
+    with open(f"{trace_dir}/run_kernel_trace.csv") as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0].split("::")[-1]
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name, int(r["Grid_Size_X"])))
+    h2d = []
+    try:
+        with open(f"{trace_dir}/run_memory_copy_trace.csv") as f:
+            h2d = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in csv.DictReader(f)
+                   if r["Direction"].endswith("HOST_TO_DEVICE")]
+    except FileNotFoundError:
+        pass
+    ev.sort()
+    pieces, cur, tail_grid = [], [], 0
+    for e in ev:
+        cur.append(e)
+        if e[2].startswith("k_reduce_tail"):
+            tail_grid = e[3]
+        elif tail_grid and "copyBuffer" in e[2]:
+            pieces.append((cur, tail_grid))
+            cur, tail_grid = [], 0
+    want = []
+    for p, grid in pieces:
+        t0, t1 = p[0][0], p[-1][1]
+        if grid // 512 != windows or any(a < t1 and b > t0 for a, b in h2d) or p[0][2] == "k_mul_generator":
+            continue
+        want.append(p)
+        if len(want) == calls:
+            break
+    per = defaultdict(float)
+    order = []
+    for p in want:
+        for e in p:
+            if e[2] not in per:
+                order.append(e[2])
+            per[e[2]] += (e[1] - e[0]) / 1e6 / len(want)
+    span = sum((p[-1][1] - p[0][0]) / 1e6 for p in want) / max(1, len(want))
+    lines = ["", f"## One device-resident call with {windows} windows (average of {len(want)} calls)", "",
+             f"First kernel start to the window sums' copy: {span:.4f} ms; kernels in order of first launch, "
+             "time summed over the launches of one call.", "", "| kernel | ms per call |", "|---|---|"]
+    lines += [f"| `{n}` | {per[n]:.4f} |" for n in order]
+    lines.append(f"| all kernels | {sum(per.values()):.4f} |")
+    with open(out_md, "a") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "run":
         run()
@@ -180,6 +315,10 @@ if __name__ == "__main__":
         print(json.dumps(summarize_valu(*sys.argv[2:6]), indent=1))
     elif len(sys.argv) >= 6 and sys.argv[1] == "trace":
         summarize_trace(*sys.argv[2:6], *(sys.argv[6:7] + [int(a) for a in sys.argv[7:8]]))
+    elif len(sys.argv) >= 6 and sys.argv[1] == "msmcall":
+        summarize_msm_call(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), sys.argv[5])
+    elif len(sys.argv) >= 6 and sys.argv[1] == "timeline":
+        summarize_timeline(*sys.argv[2:6])
     else:
         print(__doc__)
         sys.exit(2)

@@ -766,6 +766,28 @@ __global__ void k_set_count(uint32_t* counts, const uint32_t* offsets, uint32_t 
 // S = sum_b b B_b.  X' and Y' are computed by different threads (different
 // waves), so a level is two point operations deep.
 
+// The first two levels at once, one thread per block of four buckets
+// B_0..B_3 (items 4i..4i+3 of a window): X'' = 4T with T = B_0 + .. + B_3, and
+// Y'' = B_0 + 2B_1 + 3B_2 + 4B_3 = T + s_1 + s_2 + s_3 with the suffix sums
+// s_r = B_r + .. + B_3, so that S = sum_i i X''_i + sum_i Y''_i over M / 4
+// items.  Six additions and two doublings (102 products) per four buckets,
+// where two halving levels take 51 products per pair, 153 per four buckets.
+__global__ void __launch_bounds__(THREADS)
+    k_reduce_block4(const u32* __restrict__ B, uint32_t M, uint32_t n_windows, u32* __restrict__ Xo,
+                    u32* __restrict__ Yo) {
+  const uint32_t blocks = (M >> 2) * n_windows;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= blocks) return;
+  const size_t j0 = (size_t)gid * 4;  // windows are consecutive runs of M items: block gid covers 4 gid..4 gid+3
+  const G1X29 b3 = G1X29::load(B + (j0 + 3) * 32);
+  const G1X29 s2 = G1X29::load(B + (j0 + 2) * 32).add(b3);
+  const G1X29 u = s2.add(b3);  // s_2 + s_3
+  const G1X29 s1 = G1X29::load(B + (j0 + 1) * 32).add(s2);
+  const G1X29 t = G1X29::load(B + j0 * 32).add(s1);
+  t.add(s1).add(u).store(Yo + (size_t)gid * 32);
+  t.dbl().dbl().store(Xo + (size_t)gid * 32);
+}
+
 // One level over all windows: threads [0, Mo * n_windows) write X', the next
 // Mo * n_windows threads write Y' (Mo = M / 2).
 __global__ void __launch_bounds__(THREADS)
@@ -1206,7 +1228,21 @@ static hipError_t reduce_buckets(const Config& cfg, const Layout& L, char* ws, u
   const u32* Y = X;
   uint32_t M = cfg.buckets;
   if (M < 2 || (M & (M - 1))) return hipErrorInvalidValue;  // 2^(s-1) buckets, s >= 2
-  for (int rl = 0; rl < L.red_levels; rl++) {
+  int rl0 = 0;
+  if (L.red_levels >= 2) {
+    // levels 0 and 1 in one pass, into level 1's buffers
+    u32* Xo = reinterpret_cast<u32*>(ws + L.red[2]);
+    u32* Yo = reinterpret_cast<u32*>(ws + L.red[3]);
+    const uint32_t blocks = (M >> 2) * cfg.n_windows;
+    hipLaunchKernelGGL(k_reduce_block4, dim3(div_ceil32(blocks, THREADS)), dim3(THREADS), 0, stream, X, M,
+                       cfg.n_windows, Xo, Yo);
+    ZKP_CHECK(hipGetLastError());
+    X = Xo;
+    Y = Yo;
+    M >>= 2;
+    rl0 = 2;
+  }
+  for (int rl = rl0; rl < L.red_levels; rl++) {
     u32* Xo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2]);
     u32* Yo = reinterpret_cast<u32*>(ws + L.red[(rl % 2) * 2 + 1]);
     const uint32_t Mo = L.red_m[rl];

@@ -137,7 +137,7 @@ def build_fused_demo() -> str:
 
 
 def build_fused_large() -> str:
-    """Prebuilt compiler-scale fused kernels (random 700-vertex graphs) for the tests."""
+    """Prebuilt compiler-scale fused kernels (seeded random 800-vertex graphs) for the tests."""
     from . import fused
     from .runtime import Libs
 
