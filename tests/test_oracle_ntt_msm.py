@@ -93,6 +93,27 @@ def test_coset_is_evaluation_on_shifted_domain():
         assert from_mont(array_to_ints(y[j : j + 1])[0]) == acc
 
 
+def test_distribute_cyclic_table_closed_form():
+    """A table shorter than the polynomial is applied cyclically: coefficient
+    i is multiplied by powers[i % len] (the [1, zeta, zeta^2] case of
+    core/tests/ntt.rs:261-388); the NTT of the result evaluates
+    sum_i x_i zeta^(i mod 3) w^(i j), checked by big ints."""
+    log_n = 6
+    n = 1 << log_n
+    a = random_elements(n, 12)
+    zeta = pow(7, (FR_MODULUS - 1) // 3, FR_MODULUS)
+    table = [1, zeta, zeta * zeta % FR_MODULUS]
+    d = orc.distribute_powers(a, mont_arr(table))
+    xs = [from_mont(x) for x in array_to_ints(a)]
+    assert [from_mont(x) for x in array_to_ints(d)] == [x * table[i % 3] % FR_MODULUS for i, x in enumerate(xs)]
+    y = orc.ntt(d, log_n)
+    w = omega(log_n)
+    for j in (0, 1, 5, n - 1):
+        wj = pow(w, j, FR_MODULUS)
+        want = sum(x * table[i % 3] * pow(wj, i, FR_MODULUS) for i, x in enumerate(xs)) % FR_MODULUS
+        assert from_mont(array_to_ints(y[j : j + 1])[0]) == want
+
+
 def test_twiddles_and_pq_omegas():
     w = omega(12)
     tw = orc.twiddles(64, to_mont(w))
