@@ -16,6 +16,15 @@ import os
 import sys
 import time
 
+# HIP gives a process four hardware queues by default and maps further streams
+# onto them, so two streams that share a queue run one after the other.  The
+# legs below create a dozen streams (the MSM's compute and two copy streams,
+# the serving slots, the pipelined kernel's streams); with eight queues the
+# MSM's copies still overlap its sort and accumulation after the other legs
+# have made theirs.  Set before anything loads the HIP runtime.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "zkpoly-compiler_amd")
 for _p in (PKG, ROOT):
@@ -366,6 +375,7 @@ def msm_abi_leg(libs, pb, ss, n, steps, expected):
                                    hip.hipMemcpyHostToDevice, cs.raw()), "link copy")
         cs.sync()
     out = {"link_h2d_gbs": round(4 * n * 64 / (time.perf_counter() - t) / 1e9, 2)}
+    cs.destroy()
     del link
     for name, (cfg, batches) in cases.items():
         op = MSM(libs, cfg)
