@@ -137,15 +137,15 @@ def reference_choice(k: int, n: int, l_bits: int = SCALAR_BITS, p_bits: int = SC
 # MI355X (window sweep 12..18) less the modelled reduction and host combine;
 # other windows take the nearest measured neighbour's cost, 5 % dearer per bit
 # of distance.
-MI355X_ENTRY_NS = {12: 0.156, 13: 0.156, 14: 0.162, 15: 0.140, 16: 0.137, 17: 0.156, 18: 0.189}
-MI355X_REDUCE_US_PER_LEVEL = 18.0  # bucket reduction: about 2 (s - 1) dependent point operations deep
-MI355X_REDUCE_US_PER_KBUCKET = 0.235  # ... plus its work, per 1024 buckets (n_windows * 2^(s-1))
+MI355X_ENTRY_NS = {12: 0.117, 13: 0.121, 14: 0.123, 15: 0.107, 16: 0.100, 17: 0.108, 18: 0.124}
+MI355X_REDUCE_US_PER_LEVEL = 5.0  # bucket reduction: about 2 (s - 1) dependent point operations deep (quads)
+MI355X_REDUCE_US_PER_KBUCKET = 0.35  # ... plus its work, per 1024 buckets (n_windows * 2^(s-1))
 MI355X_HOST_US = 60.0  # host combine (Horner over the window sums, one inversion) per batch
-MI355X_PART_US = 150.0  # fixed cost of one part's sort and accumulation (about 25 launches, their tails)
+MI355X_PART_US = 250.0  # fixed cost of one part (about 25 launches, their tails, the partial levels of a smaller part)
 MI355X_SMALL_PART = 2 ** 18  # below this part size per-entry costs grow (fewer waves per launch)
 MI355X_LINK_GBS = 55.0  # host -> device copy rate from pinned memory
-MI355X_MERGE_US_PER_KBUCKET = 0.25  # a later part's scratch buckets reset and merged, per 1024 buckets
-MI355X_SORT_SHARE = 0.27  # share of the per-entry cost spent sorting (needs the scalars only)
+MI355X_MERGE_US_PER_KBUCKET = 0.15  # a later part's scratch buckets reset and merged, per 1024 buckets
+MI355X_SORT_SHARE = 0.24  # share of the per-entry cost spent sorting (needs the scalars only)
 
 
 def _entry_ns(s: int) -> float:
