@@ -47,7 +47,7 @@ static bool below(const Fq29& v, int k) {
     if (v.l[i] > Fq29::MASK) return false;
   int64_t c = 0;
   for (int i = 0; i < 9; i++) {
-    c += (int64_t)v.l[i] - (int64_t)k * Fq29::P[i];
+    c += (int64_t)v.l[i] - (int64_t)k * Fq29Params::P[i];
     if (i < 8) c >>= 29;
   }
   return c < 0;

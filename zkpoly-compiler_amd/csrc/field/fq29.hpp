@@ -1,5 +1,7 @@
 // BN254 Fq in nine 29-bit limbs with Montgomery radix R' = 2^261: the
-// arithmetic of the MSM's buckets (host + device).
+// arithmetic of the MSM's buckets (host + device).  The limb layout and the
+// bounds are written for any modulus below 2^254 (F29<Params>); the NTT keeps
+// the 8 x 32-bit product of bn254.hpp (see the note on F29 below).
 //
 // Why 29 bits: a 29 x 29-bit limb product is below 2^58, so one column of the
 // product-scanning multiplication (at most nine a_i b_j and nine m_i p_j
@@ -32,10 +34,8 @@
 
 namespace zkp {
 
-struct Fq29 {
-  u32 l[9];
-
-  static constexpr u32 MASK = (1u << 29) - 1;
+// The base field q (the MSM's buckets).
+struct Fq29Params {
   static constexpr u32 P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
                                0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
   static constexpr u32 INV = 0x04866389u;  // -p^-1 mod 2^29
@@ -43,21 +43,33 @@ struct Fq29 {
                                  0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};  // 2^261 mod p
   static constexpr u32 K256[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u,
                                   0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};  // 2^256 mod p
-  static constexpr u32 P_TOP = 0x0030644eu;  // p >> 232
+};
 
-  ZKP_HD static Fq29 zero() {
-    Fq29 r;
+// The NTT's twiddle product was tried through this form too (x read as is, the
+// twiddle 5 bits up, the result packed back): 267 instructions against 295, but
+// 171 v_mad_u64_u32 against 136, and the 2^24 transform ran 2.21 ms against
+// 2.14 ms -- the multiply-adds, not the instruction count, set its pace.
+template <class Params>
+struct F29 {
+  u32 l[9];
+
+  static constexpr u32 MASK = (1u << 29) - 1;
+  static constexpr u32 INV = Params::INV;
+  static constexpr u32 P_TOP = Params::P[8];  // p >> 232
+
+  ZKP_HD static F29 zero() {
+    F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) r.l[i] = 0;
     return r;
   }
-  ZKP_HD static Fq29 konst(const u32 (&c)[9]) {
-    Fq29 r;
+  ZKP_HD static F29 konst(const u32 (&c)[9]) {
+    F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) r.l[i] = c[i];
     return r;
   }
-  ZKP_HD static Fq29 one() { return konst(ONE); }
+  ZKP_HD static F29 one() { return konst(Params::ONE); }
 
   ZKP_HD bool all_zero() const {
     u32 acc = 0;
@@ -68,8 +80,8 @@ struct Fq29 {
 
   // (w << SHIFT) for a 256-bit little-endian word array w, SHIFT <= 5.
   template <int SHIFT>
-  ZKP_HD static Fq29 from_words(const u32 (&w)[8]) {
-    Fq29 r;
+  ZKP_HD static F29 from_words(const u32 (&w)[8]) {
+    F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
       const int s = 29 * i - SHIFT;  // first source bit of limb i
@@ -108,13 +120,13 @@ struct Fq29 {
     w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
   }
   // an element of this form packed in HBM
-  __device__ __forceinline__ static Fq29 load(const u32* p) {
+  __device__ __forceinline__ static F29 load(const u32* p) {
     u32 w[8];
     load_words(p, w);
     return from_words<0>(w);
   }
   // an element in the form of bn254.hpp (canonical x 2^256), as x 2^261 below 32 p
-  __device__ __forceinline__ static Fq29 load_mont256(const u32* p) {
+  __device__ __forceinline__ static F29 load_mont256(const u32* p) {
     u32 w[8];
     load_words(p, w);
     return from_words<5>(w);
@@ -128,9 +140,9 @@ struct Fq29 {
   }
 
   // Montgomery product a b 2^-261 (bounds above).
-  ZKP_HD Fq29 operator*(const Fq29& b) const {
+  ZKP_HD F29 operator*(const F29& b) const {
     u32 m[9];
-    Fq29 r;
+    F29 r;
     u64 acc = 0;
 #pragma unroll
     for (int k = 0; k < 17; k++) {
@@ -142,11 +154,11 @@ struct Fq29 {
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         const int j = k - i;
-        if (i < k && j >= 1 && j <= 8) acc += (u64)m[i] * P[j];
+        if (i < k && j >= 1 && j <= 8) acc += (u64)m[i] * Params::P[j];
       }
       if (k < 9) {
         m[k] = ((u32)acc * INV) & MASK;
-        acc += (u64)m[k] * P[0];  // clears the low 29 bits
+        acc += (u64)m[k] * Params::P[0];  // clears the low 29 bits
       } else {
         r.l[k - 9] = (u32)acc & MASK;
       }
@@ -156,16 +168,16 @@ struct Fq29 {
     return r;
   }
 
-  ZKP_HD Fq29 operator+(const Fq29& b) const {
-    Fq29 r;
+  ZKP_HD F29 operator+(const F29& b) const {
+    F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) r.l[i] = l[i] + b.l[i];
     return r;
   }
 
   // Carries every limb above 29 bits into the next (value unchanged).
-  ZKP_HD Fq29 norm() const {
-    Fq29 r;
+  ZKP_HD F29 norm() const {
+    F29 r;
     u32 c = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -184,7 +196,7 @@ struct Fq29 {
     constexpr Lifted() : l{} {
       u64 c = 0;
       for (int i = 0; i < 9; i++) {
-        c += (u64)K * P[i];
+        c += (u64)K * Params::P[i];
         l[i] = i < 8 ? (u32)(c & MASK) : (u32)c;
         c >>= 29;
       }
@@ -197,9 +209,9 @@ struct Fq29 {
 
   // this + K p - b, normalised; b < K p with limbs below 2^31 - 4, limbs of this below 2^30.
   template <int K>
-  ZKP_HD Fq29 sub(const Fq29& b) const {
+  ZKP_HD F29 sub(const F29& b) const {
     constexpr Lifted<K> kp{};
-    Fq29 r;
+    F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) r.l[i] = l[i] + kp.l[i] - b.l[i];
     return r.norm();
@@ -210,12 +222,12 @@ struct Fq29 {
   // but a 2^-29 share of the nonzero values; the rest compare every limb.
   ZKP_HD bool is_zero_modp() const {
     const u32 q = l[8] / P_TOP;
-    if (((u32)((u64)q * P[0]) ^ l[0]) & MASK) return false;
+    if (((u32)((u64)q * Params::P[0]) ^ l[0]) & MASK) return false;
     u64 c = 0;
     u32 diff = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      c += (u64)q * P[i];
+      c += (u64)q * Params::P[i];
       diff |= (i < 8 ? ((u32)c & MASK) : (u32)c) ^ l[i];
       c >>= 29;
     }
@@ -225,13 +237,13 @@ struct Fq29 {
   // A normalised value below 2^261 brought below 2p: q' = top / (p_top + 1)
   // never exceeds q = floor(x / p), and is at least q - 1 since
   // top > q p_top - 1 and (q + 1) / (p_top + 1) < 1 for every such x.
-  ZKP_HD Fq29 reduce_2p() const {
+  ZKP_HD F29 reduce_2p() const {
     const u32 q = l[8] / (P_TOP + 1);
-    Fq29 r;
+    F29 r;
     int64_t c = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      c += (int64_t)l[i] - (int64_t)((u64)q * P[i]);
+      c += (int64_t)l[i] - (int64_t)((u64)q * Params::P[i]);
       r.l[i] = i < 8 ? (u32)c & MASK : (u32)c;
       c >>= 29;  // arithmetic: the borrow
     }
@@ -239,13 +251,13 @@ struct Fq29 {
   }
 
   // The canonical value (below p) of a normalised value.
-  ZKP_HD Fq29 canonical() const {
-    const Fq29 r = reduce_2p();
-    Fq29 t;
+  ZKP_HD F29 canonical() const {
+    const F29 r = reduce_2p();
+    F29 t;
     int64_t c = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      c += (int64_t)r.l[i] - (int64_t)P[i];
+      c += (int64_t)r.l[i] - (int64_t)Params::P[i];
       t.l[i] = i < 8 ? (u32)c & MASK : (u32)c;
       if (i < 8) c >>= 29;
     }
@@ -255,8 +267,10 @@ struct Fq29 {
   // x 2^261 (this form, below 16 p: the product by K256 stays below 2p) -> canonical x 2^256,
   // packed: the form of bn254.hpp.
   ZKP_HD void to_mont256(u32 (&w)[8]) const {
-    ((*this) * konst(K256)).canonical().to_words(w);
+    ((*this) * konst(Params::K256)).canonical().to_words(w);
   }
 };
+
+using Fq29 = F29<Fq29Params>;
 
 }  // namespace zkp
