@@ -7,6 +7,8 @@ window sweep 12..18, checked through the closed form (sum k_i s_i) G.  Edge
 cases: n = 1, all-zero scalars (identity (0, 0)), repeated and opposite bases
 (the equal-point / inverse-point branches), precomputed tables.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -177,6 +179,42 @@ def test_reference_abi_runs_and_stages(libs, stream, batches, bpr, parts, ss, sp
     op.get_fn().f([GpuBuffer(buf.addr, size)] + results, [PointArray(pts_ptr, n)] + arrays, lambda c: c)
     for b in range(batches):
         assert list(results[b].data) == list(closed_form(ks, scal[b])), f"batch {b}"
+    pool.destroy()
+
+
+@pytest.mark.parametrize("pt_shift,sc_shift,parts", [(16, 32, 1), (8, 4, 1), (16, 32, 2)])
+def test_reference_abi_pinned_offsets(libs, stream, pt_shift, sc_shift, parts):
+    """Pinned host arrays that start pt_shift / sc_shift bytes into a pool
+    block, two batches in one run: in one part, 16-byte aligned ones are
+    pulled into HBM by the library's copy kernel and others go through
+    hipMemcpyAsync; in two parts every copy is a DMA copy.  Against the closed
+    form."""
+    from zkpoly_amd.pool import CpuMemoryPool
+
+    n = 6001
+    ks = random_elements(n, 8700 + pt_shift)
+    pb = gpu_bases(libs, stream, ks)
+    pool = CpuMemoryPool(20, 64, libs)
+    pts_ptr = pool.allocate(n * 64 + 64) + pt_shift
+    pv = (ctypes.c_uint8 * (n * 64)).from_address(pts_ptr)
+    np.frombuffer(pv, dtype=np.uint8)[:] = pb.download(np.uint64, n * 8).view(np.uint8)
+    scal, arrays = [], []
+    for b in range(2):
+        ss = random_elements(n, 8800 + sc_shift + b)
+        sc_ptr = pool.allocate(n * 32 + 64) + sc_shift
+        sv = (ctypes.c_uint8 * (n * 32)).from_address(sc_ptr)
+        np.frombuffer(sv, dtype=np.uint8)[:] = ss.reshape(-1).view(np.uint8)
+        scal.append(ss)
+        arrays.append(ScalarArray(n, sc_ptr, "cpu"))
+    cfg = MsmConfig(window_size=12, target_window=22, parts=parts, stage_scalars=2, stage_points=2, batch_per_run=2)
+    op = MSM(libs, cfg)
+    size = op.get_buffer_size(n)[0]
+    buf = hip.DeviceBuffer(size)
+    res = [Point(), Point()]
+    print(f"msm pinned offsets points +{pt_shift} scalars +{sc_shift} parts {parts}", flush=True)
+    op.get_fn().f([GpuBuffer(buf.addr, size)] + res, [PointArray(pts_ptr, n)] + arrays, lambda c: c)
+    for b in range(2):
+        assert list(res[b].data) == list(closed_form(ks, scal[b])), f"batch {b}"
     pool.destroy()
 
 

@@ -737,6 +737,35 @@ __global__ void __launch_bounds__(THREADS) k_fill_identity(u32* pts, uint64_t co
     reinterpret_cast<uint4*>(pts)[i] = z;
 }
 
+// dst[0, n16) = src[0, n16) in 16-byte words, src being pinned host memory
+// mapped into the device's address space: the host-to-device copies of the
+// drop-in msm's points and scalars.  A grid of PULL_BLOCKS workgroups with four
+// loads in flight per lane keeps the PCIe link full (57 GB/s measured on
+// MI355X for 64 MiB, the link's rate) on a small share of the CUs, so the
+// compute stream's sort runs beside it.  DMA copies reach the same rate on some
+// copy engines and about half of it on others, and which engine serves a stream
+// depends on what the process did before (a one-part 2^20 call took 3.8 ms or
+// 4.9 ms that way); the kernel's rate does not.
+constexpr uint32_t PULL_BLOCKS = 64;
+constexpr int PULL_UNROLL = 4;
+__global__ void __launch_bounds__(THREADS) k_pull_host(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                       uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * THREADS * PULL_UNROLL;
+  for (uint64_t base = (uint64_t)blockIdx.x * THREADS * PULL_UNROLL + threadIdx.x; base < n16; base += stride) {
+    uint4 v[PULL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PULL_UNROLL; u++) {
+      const uint64_t i = base + (uint64_t)u * THREADS;
+      if (i < n16) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < PULL_UNROLL; u++) {
+      const uint64_t i = base + (uint64_t)u * THREADS;
+      if (i < n16) dst[i] = v[u];
+    }
+  }
+}
+
 // dst[i] += src[i] over H buckets (XYZZ): a later part's buckets, accumulated
 // into a scratch set of their own, join the batch's buckets in one pass with
 // no divergence, instead of every run of that part reading and adding its
@@ -1414,6 +1443,27 @@ struct ResLease {
   }
 };
 
+// Host-to-device copy of `bytes` on stream s.  With `pull`, pinned host memory
+// that the device addresses at the host's own virtual address (hipHostMalloc'd
+// or registered memory) is pulled by k_pull_host when both ends are 16-byte
+// aligned; any other source (pageable memory), and every copy without `pull`,
+// goes through hipMemcpyAsync.
+static hipError_t copy_in(void* dst, const void* src, size_t bytes, hipStream_t s, bool pull) {
+  if (bytes == 0) return hipSuccess;
+  if (!pull) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  hipPointerAttribute_t a{};
+  const hipError_t q = hipPointerGetAttributes(&a, src);
+  if (q != hipSuccess) (void)hipGetLastError();  // an unregistered pointer may be reported as an error
+  const bool pinned = q == hipSuccess && a.type == hipMemoryTypeHost && a.devicePointer != nullptr &&
+                      a.devicePointer == a.hostPointer;
+  if (pinned && bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0) {
+    hipLaunchKernelGGL(k_pull_host, dim3(PULL_BLOCKS), dim3(THREADS), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), (uint64_t)(bytes / 16));
+    return hipGetLastError();
+  }
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+}
+
 // One card's share: points [c_off, c_off + c_len) of every table, all
 // batches; adds the card's partial result of batch b into totals[b].
 //
@@ -1437,6 +1487,12 @@ static hipError_t msm_card(const Config& cfg, const HostPlan& P, unsigned card, 
   auto pts = [&](uint32_t st) { return reinterpret_cast<u32*>(buf + st * P.pts_bytes); };
   auto scs = [&](uint32_t st) { return reinterpret_cast<u32*>(buf + P.stage_points * P.pts_bytes + st * P.sc_bytes); };
   const uint32_t nparts = c_len ? (uint32_t)((c_len + P.part_len - 1) / P.part_len) : 0;
+  // The copy kernel needs room on the CUs.  With one part the points cross the
+  // link beside the sort only; with several, each part's copy runs beside the
+  // previous part's accumulation, whose waves hold every CU's registers, and
+  // the kernel would wait for them (2^20 points, window 16, four parts: 4.6 ms
+  // with the kernel, 3.9 ms with DMA), so those copies go to the DMA engines.
+  const bool pull = nparts == 1;
   auto part_off = [&](uint32_t p) { return c_off + (uint64_t)p * P.part_len; };
   auto part_len = [&](uint32_t p) {
     const uint64_t o = part_off(p), end = c_off + c_len;
@@ -1465,8 +1521,7 @@ static hipError_t msm_card(const Config& cfg, const HostPlan& P, unsigned card, 
     if (after != nullptr) ZKP_CHECK(hipStreamWaitEvent(R.cp, after, 0));
     const uint64_t off = part_off(p), n = part_len(p);
     for (uint32_t t = 0; t < cfg.n_precompute; t++)
-      ZKP_CHECK(hipMemcpyAsync(pts(st) + (size_t)t * n * 16, h_points[t] + off * 16, n * 64, hipMemcpyHostToDevice,
-                               R.cp));
+      ZKP_CHECK(copy_in(pts(st) + (size_t)t * n * 16, h_points[t] + off * 16, n * 64, R.cp, pull));
     ZKP_CHECK(hipEventRecord(R.pt_ready[st], R.cp));
     held[st] = p;
     last_use[st] = ++clock;
@@ -1485,7 +1540,7 @@ static hipError_t msm_card(const Config& cfg, const HostPlan& P, unsigned card, 
       for (uint32_t j = 0; j < nb; j++) {
         const uint32_t ss = sc_next++ % P.stage_scalars;
         ZKP_CHECK(hipStreamWaitEvent(R.cs, R.sc_free[ss], 0));
-        ZKP_CHECK(hipMemcpyAsync(scs(ss), h_scalars[r0 + j] + off * 8, n * 32, hipMemcpyHostToDevice, R.cs));
+        ZKP_CHECK(copy_in(scs(ss), h_scalars[r0 + j] + off * 8, n * 32, R.cs, pull));
         ZKP_CHECK(hipEventRecord(R.sc_ready[ss], R.cs));
         if (j == 0) {
           // the part's points after its first scalars (resident parts are not copied again), then the next
