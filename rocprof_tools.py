@@ -213,7 +213,10 @@ def summarize_timeline(trace_dir: str, title: str, command: str, out_md: str) ->
     with open(f"{trace_dir}/run_kernel_trace.csv") as f:
         for r in csv.DictReader(f):
             name = r["Kernel_Name"].split("(")[0].split("::")[-1]
-            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), "kernel", name))
+            # the library's copy kernel (pinned host memory read by the device) is a host-to-device copy
+            kind = "copy" if name.startswith("k_pull_host") else "kernel"
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kind,
+                       "host_to_device (k_pull_host)" if kind == "copy" else name))
     with open(f"{trace_dir}/run_memory_copy_trace.csv") as f:
         for r in csv.DictReader(f):
             kind = r["Direction"].replace("MEMORY_COPY_", "").lower()
@@ -231,7 +234,7 @@ def summarize_timeline(trace_dir: str, title: str, command: str, out_md: str) ->
             cur, reduced = [], False
     rows = []
     for c in calls:
-        h2d = [(a, b) for a, b, k, n in c if k == "copy" and n == "host_to_device"]
+        h2d = [(a, b) for a, b, k, n in c if k == "copy" and n.startswith("host_to_device")]
         ker = [(a, b) for a, b, k, n in c if k == "kernel"]
         parts = sum(1 for e in c if e[3] == "k_part_count")
         if not h2d or not parts:
