@@ -442,6 +442,15 @@ def batched_leg(libs, stream, steps, with_cpu):
         gm.msm_results()
 
     batched_m = time_loop(stream, batched_with_msms, steps, 2)
+    # the same graph with every MSM after the fused kernel on the capturing stream (no parallel branches)
+    gs = wm.capture(branches=0)
+
+    def batched_serial():
+        gs.launch(stream)
+        stream.sync()
+        gs.msm_results()
+
+    serial_m = time_loop(stream, batched_serial, steps, 2)
     work = wm.total_elements + wm.total_msm_points
     res_plain = wm.run_plain()
     stream.sync()
@@ -457,6 +466,8 @@ def batched_leg(libs, stream, steps, with_cpu):
         "plain_gelem_s": round(work / plain_m / 1e6, 4),
         "batched_gelem_s": round(work / batched_m / 1e6, 4),
         "speedup": round(plain_m / batched_m, 4),
+        "msm_graph_branches": gm.branches,
+        "batched_one_branch_ms": round(serial_m, 4),
         "msm_results_equal_plain": [list(a) for a in res_plain] == [list(b) for b in gm.msm_results()],
     }
     if with_cpu:

@@ -76,16 +76,18 @@ def msm_closed_form(w, j):
     return list(orc.g1_mul_canonical(orc.generator(), total))
 
 
-def test_graph_with_msms_replays_bit_exact(libs, stream, kernel):
+@pytest.mark.parametrize("branches", [4, 0])
+def test_graph_with_msms_replays_bit_exact(libs, stream, kernel, branches):
     """NTTs, the fused kernel and MSMs of uneven sizes (2^10..2^16 points, each
-    its own window) in one graph; three replays with eager work between them,
-    every NTT, fused output and MSM result checked, and the MSMs equal to the
-    plain synchronous calls."""
+    its own window) in one graph, the MSMs on parallel branches beside the
+    transforms (or all on the capturing stream); three replays with eager work
+    between them, every NTT, fused output and MSM result checked, and the MSMs
+    equal to the plain synchronous calls."""
     w = Config4Workload(libs, stream, kernel, count=10, lo=6, hi=15, seed=11, msm_count=6, msm_lo=10, msm_hi=16)
-    print("msm sizes", w.msm_log_lens, "windows", [m.window for m in w.msms], flush=True)
+    print("msm sizes", w.msm_log_lens, "windows", [m.window for m in w.msms], "branches", branches, flush=True)
     want = [expected(w, i) for i in range(len(w.lens))]
     want_msm = [msm_closed_form(w, j) for j in range(len(w.msms))]
-    graph = w.capture()
+    graph = w.capture(branches)
     for replay in range(3):
         w.reset_inputs()
         for m in w.msms:

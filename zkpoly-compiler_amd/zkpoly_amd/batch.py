@@ -95,17 +95,53 @@ class FusedBatch:
                   self.kernel.meta.name + "_batched")
 
 
-class ProofStepBatch:
-    """NTT batch + fused batch + MSMs captured as one hipGraph."""
+def msm_branches(lengths: list[int], branches: int) -> list[list[int]]:
+    """MSM indices per graph branch: largest first, each to the branch with the
+    fewest points so far (ties to the lower branch), kept in index order within
+    a branch."""
+    loads = [0] * branches
+    out = [[] for _ in range(branches)]
+    for j in sorted(range(len(lengths)), key=lambda j: (-lengths[j], j)):
+        b = min(range(branches), key=lambda k: (loads[k], k))
+        out[b].append(j)
+        loads[b] += lengths[j]
+    return [sorted(x) for x in out if x]
 
-    def __init__(self, stream: hip.Stream, ntt: NttBatch, fused: FusedBatch, msms=()):
+
+class ProofStepBatch:
+    """NTT batch + fused batch + MSMs captured as one hipGraph.
+
+    The MSMs do not depend on the transforms, and most of an MSM below 2^18
+    points is latency-bound (sort passes over few keys, the narrow reduction
+    levels), so they run on `branches` parallel branches of the graph beside
+    the NTT and fused kernels: the capture forks from the capturing stream
+    onto side streams through an event and joins back through one event per
+    branch."""
+
+    def __init__(self, stream: hip.Stream, ntt: NttBatch, fused: FusedBatch, msms=(), branches: int = 4):
+        """branches = 0 puts every MSM on the capturing stream after the fused kernel."""
         self.ntt, self.fused, self.msms = ntt, fused, list(msms)
+        groups = msm_branches([m.length for m in self.msms], branches) if self.msms and branches > 0 else []
+        self.branches = len(groups)  # parallel MSM branches in the graph (0: MSMs on the capturing stream)
+        self._side = [hip.Stream(stream.get_device()) for _ in groups]
+        self._fork = hip.Event()
+        self._join = [hip.Event() for _ in groups]
 
         def fill(s):
+            if groups:
+                self._fork.record(s)
+            for side, group in zip(self._side, groups):
+                side.wait(self._fork)
+                for j in group:
+                    self.msms[j].enqueue(side)
             ntt.run(s)
             fused.run(s)
-            for m in self.msms:
-                m.enqueue(s)
+            if not groups:
+                for m in self.msms:
+                    m.enqueue(s)
+            for side, ev in zip(self._side, self._join):
+                ev.record(side)
+                s.wait(ev)
 
         self.graph = hip.Graph(stream, fill)
 
@@ -243,8 +279,8 @@ class Config4Workload:
                                         m.workspace, s))
         return out
 
-    def capture(self) -> ProofStepBatch:
-        return ProofStepBatch(self.stream, self.ntt_batch, self.fused_batch, self.msms)
+    def capture(self, branches: int = 4) -> ProofStepBatch:
+        return ProofStepBatch(self.stream, self.ntt_batch, self.fused_batch, self.msms, branches)
 
     def results(self, i: int) -> tuple[np.ndarray, np.ndarray]:
         n = self.lens[i]

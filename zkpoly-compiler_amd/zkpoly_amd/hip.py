@@ -57,6 +57,7 @@ def _lib():
         h.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
         h.hipEventDestroy.argtypes = [vp]
         h.hipEventRecord.argtypes = [vp, vp]
+        h.hipStreamWaitEvent.argtypes = [vp, vp, ctypes.c_uint]
         h.hipEventSynchronize.argtypes = [vp]
         h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(i)]
@@ -219,6 +220,10 @@ class Stream:
 
     def sync(self) -> None:
         check(_lib().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def wait(self, event: "Event") -> None:
+        """Later work on this stream waits for `event` (under capture: a graph edge)."""
+        check(_lib().hipStreamWaitEvent(self.handle, event.handle, 0), "hipStreamWaitEvent")
 
     def destroy(self) -> None:
         if self.handle.value:
