@@ -320,6 +320,21 @@ def test_narrow_and_wide_windows(libs, stream, window):
     assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss))
 
 
+@pytest.mark.parametrize("window", [2, 3, 13, 16, 21])
+def test_carries_through_every_window(libs, stream, window):
+    """Every fifth scalar r - 1, whose signed digits carry through every
+    window into the top one, among uniform scalars, at 2^16 - 1 and 2^16
+    points: odd window counts (13: 20 windows, 21: 13) and the extra top
+    window of 2 and 3 bits, against the closed form."""
+    for n in ((1 << 16) - 1, 1 << 16):
+        ks = random_elements(n, 6200 + window)
+        ss = random_elements(n, 6300 + window)
+        ss[::5] = mont_limbs(FR_MODULUS - 1)
+        pb = gpu_bases(libs, stream, ks)
+        print(f"msm carries n={n} window={window}", flush=True)
+        assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss)), n
+
+
 @pytest.mark.parametrize("value", [1, FR_MODULUS - 1, 12345])
 def test_all_scalars_equal(libs, stream, value):
     """Every scalar the same: each window's entries land in one bucket, one sort
