@@ -800,6 +800,37 @@ def traffic_from_profiles(kernel_key):
         return None
 
 
+# wave64 VALU instructions the chip can issue per second: 256 CUs x 4 SIMDs, each
+# taking 4 cycles for a 64-lane instruction on its 16 lanes, at 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def valu_issue(ms_step, dit_avg_ms):
+    """VALU issue rate of the NTT step and of `dit_pass_r4` against VALU_ISSUE_PEAK: wave-level
+    VALU instructions per launch from the committed SQ_INSTS_VALU summary of the same transform
+    (its last step: head, pass, pass), divided by this run's launch times."""
+    path = os.path.join(ROOT, "profiles", "pmc_insts_ntt_2e24.json")
+    try:
+        with open(path) as f:
+            last = json.load(f)["last"][-3:]
+    except (OSError, ValueError, KeyError):
+        return None
+    if [e[1] for e in last] != ["bitrev_head_stream", "dit_pass_r4", "dit_pass_r4"]:
+        return None
+    step = sum(e[3] for e in last)
+    dit = (last[1][3] + last[2][3]) / 2
+    return {
+        "bound": "valu",
+        "unit": "wave64 VALU instructions/s",
+        "peak": VALU_ISSUE_PEAK,
+        "step_insts": step,
+        "step_frac": round(step / (ms_step * 1e-3) / VALU_ISSUE_PEAK, 4),
+        "dit_pass_r4_insts_per_launch": dit,
+        "dit_pass_r4_frac": round(dit / (dit_avg_ms * 1e-3) / VALU_ISSUE_PEAK, 4),
+        "source": "profiles/pmc_insts_ntt_2e24.json (SQ_INSTS_VALU), launch times of this run",
+    }
+
+
 def valu_from_profiles(kernels):
     """VALUBusy / VALUUtilization per kernel from the committed rocprofv3 summary, if present."""
     path = os.path.join(ROOT, "profiles", "pmc_valu.json")
@@ -897,12 +928,14 @@ def main():
             "launches": len(dit),
         },
         "int_mul": {
-            "note": "the NTT is integer-multiply bound: 136 v_mad_u64_u32 per Montgomery product, "
-                    "n/2 products per stage, rate over the whole step",
+            "note": "v_mad_u64_u32 rate over the whole step: 136 per Montgomery product, n/2 products per "
+                    "stage; they are about half of the product's VALU instructions, so the binding rate is "
+                    "the VALU issue in valu_issue",
             "achieved_mads_per_s": float(f"{int_rate:.4e}"),
             "peak_mads_per_s": INT_MUL_PEAK,
             "frac": round(int_rate / INT_MUL_PEAK, 4),
         },
+        "valu_issue": valu_issue(ms_step, dit_avg_ms) if k == 24 else None,
         "bitrev_head_avg_ms": round(sum(per_kind[0]) / max(1, len(per_kind[0])), 5),
         "valu_rocprof": valu_from_profiles(["dit_pass_r4", "bitrev_head_stream", "k_accumulate0", "k_close_runs",
                                             "k_reduce_level", "k_reduce_tail", "k_seg_scatter"]),

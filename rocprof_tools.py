@@ -28,8 +28,10 @@ kernels (the end-to-end msm symbol): their span, the time the copy engines
 and the kernels were busy, and how much of the copy time overlapped kernels;
 then lists the events of the last piece with the most parts.
 
-`python3 rocprof_tools.py counter CSV COUNTER TITLE COMMAND OUT.md` tabulates
-the mean of one counter per kernel and grid size from a `rocprofv3 --pmc` run.
+`python3 rocprof_tools.py counter CSV COUNTER TITLE COMMAND OUT.md [LAST]` tabulates
+the mean of one counter per kernel and grid size from a `rocprofv3 --pmc` run,
+and with LAST the values of the last LAST dispatches in order (also written
+as JSON beside OUT.md).
 
 `python3 rocprof_tools.py msmcall DIR WINDOWS CALLS OUT.md` appends to OUT.md
 the average time per kernel of the first CALLS device-resident MSM calls
@@ -265,19 +267,30 @@ def summarize_timeline(trace_dir: str, title: str, command: str, out_md: str) ->
         f.write("\n".join(lines) + "\n")
 
 
-def summarize_counter(csv_path: str, counter: str, title: str, command: str, out_md: str) -> None:
+def summarize_counter(csv_path: str, counter: str, title: str, command: str, out_md: str, last: int = 0) -> None:
     vals = defaultdict(list)
+    seq = []
     with open(csv_path) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] != counter:
                 continue
             name = r["Kernel_Name"].split("(")[0].split("::")[-1]
             vals[(name, int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+            seq.append((int(r["Dispatch_Id"]), name, int(r["Grid_Size"]), float(r["Counter_Value"])))
     lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
              f"Mean `{counter}` per dispatch, by kernel and grid size (threads).", "",
              "| kernel | grid | dispatches | mean |", "|---|---|---|---|"]
     for (name, grid), v in sorted(vals.items(), key=lambda kv: (-sum(kv[1]) / len(kv[1]), kv[0])):
         lines.append(f"| `{name}` | {grid} | {len(v)} | {sum(v) / len(v):.4g} |")
+    if last > 0:
+        lines += ["", f"The last {last} dispatches, in order:", "", f"| dispatch | kernel | grid | `{counter}` |",
+                  "|---|---|---|---|"]
+        for d, name, grid, v in sorted(seq)[-last:]:
+            lines.append(f"| {d} | `{name}` | {grid} | {v:.6g} |")
+        # the same dispatches for bench.py (beside OUT.md, .json)
+        with open(os.path.splitext(out_md)[0] + ".json", "w") as f:
+            json.dump({"counter": counter, "last": [[d, name, grid, v] for d, name, grid, v in sorted(seq)[-last:]]},
+                      f, indent=1)
     with open(out_md, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -339,7 +352,7 @@ if __name__ == "__main__":
     elif len(sys.argv) >= 6 and sys.argv[1] == "trace":
         summarize_trace(*sys.argv[2:6], *(sys.argv[6:7] + [int(a) for a in sys.argv[7:8]]))
     elif len(sys.argv) >= 7 and sys.argv[1] == "counter":
-        summarize_counter(*sys.argv[2:7])
+        summarize_counter(*sys.argv[2:7], *[int(a) for a in sys.argv[7:8]])
     elif len(sys.argv) >= 6 and sys.argv[1] == "msmcall":
         summarize_msm_call(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), sys.argv[5])
     elif len(sys.argv) >= 6 and sys.argv[1] == "timeline":
