@@ -266,6 +266,58 @@ def test_workload_2_20_other_windows(libs, stream, window):
     assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss)), window
 
 
+def test_reference_abi_concurrent_calls(libs, stream):
+    """Three threads call the drop-in msm at once (ctypes drops the GIL for the
+    call), each with its own configuration, device buffer and host arrays:
+    one part from pinned pool memory (the copy kernel), one part from pageable
+    memory, three parts.  Each call takes its own set of streams, events and
+    pinned sums from the library's free list; every result against the
+    closed form."""
+    import threading
+
+    from zkpoly_amd.pool import CpuMemoryPool
+
+    n = (1 << 16) + 5
+    ks = random_elements(n, 9400)
+    pb = gpu_bases(libs, stream, ks)
+    host_pts = np.ascontiguousarray(pb.download(np.uint64, n * 8).reshape(n, 8))
+    pool = CpuMemoryPool(22, 64, libs)
+    pin_pts = pool.allocate(n * 64)
+    pool.view(pin_pts, n * 64)[:] = host_pts.reshape(-1)
+    jobs = []
+    for j, (window, parts, pinned) in enumerate(((16, 1, True), (13, 1, False), (14, 3, False))):
+        ss = random_elements(n, 9500 + j)
+        if pinned:
+            sp = pool.allocate(n * 32)
+            pool.view(sp, n * 32)[:] = ss.reshape(-1)
+            arrays = [PointArray(pin_pts, n), ScalarArray(n, sp, "cpu")]
+        else:
+            arrays = [PointArray(host_pts.ctypes.data, n), ScalarArray(n, ss.ctypes.data, "cpu")]
+        op = MSM(libs, MsmConfig(window_size=window, target_window=-(-254 // window), parts=parts))
+        size = op.get_buffer_size(n)[0]
+        jobs.append((op, hip.DeviceBuffer(size), size, Point(), arrays, ss))
+    errors = []
+
+    def run(job):
+        op, buf, size, res, arrays, _ = job
+        try:
+            for _ in range(3):
+                op.get_fn().f([GpuBuffer(buf.addr, size), res], arrays, lambda c: c)
+        except Exception as e:  # reported below, on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(job,)) for job in jobs]
+    print("msm concurrent calls: 3 threads", flush=True)
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for j, (_, _, _, res, _, ss) in enumerate(jobs):
+        assert list(res.data) == list(closed_form(ks, ss)), j
+    pool.destroy()
+
+
 @pytest.mark.parametrize("window,target,parts", [(16, 16, 1), (16, 16, 2), (13, 10, 3)])
 def test_reference_abi_2_20(libs, stream, window, target, parts):
     """The drop-in msm symbol at 2^20: host points and two batches of host
