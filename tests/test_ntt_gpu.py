@@ -326,3 +326,54 @@ def test_extended_coset_4x_of_2_22(libs, stream):
     stream.sync()
     want = orc.ntt(orc.distribute_powers(padded, orc.twiddles(1 << ek, to_mont(7))), ek, THREADS)
     assert np.array_equal(download(x, buf), want)
+
+
+def test_concurrent_transforms_while_tuning_changes(libs):
+    """Four threads transform their own 2^18 inputs on their own streams, three
+    times each, while a fifth thread keeps changing the pass depth of 2^18
+    transforms (zkp_ntt_set_tuning).  Every plan computes the same network, and
+    a launch reads the setting once, so every result equals the oracle."""
+    import threading
+
+    k = 18
+    n = 1 << k
+    tw_buf, tw = upload(twiddle_table(libs, orc.omega(k), n))
+    inputs = [random_elements(n, 1800 + j) for j in range(4)]
+    want = [orc.ntt(x, k, THREADS) for x in inputs]
+    streams = [hip.Stream(0) for _ in range(4)]
+    bufs = [upload(x) for x in inputs]
+    ntt = SsipNtt(libs).get_fn()
+    done = threading.Event()
+    errors, results = [], [None] * 4
+
+    def transform(j):
+        try:
+            buf, x = bufs[j]
+            for rep in range(3):
+                buf.upload(inputs[j])
+                ntt.f([x], [tw, streams[j]])
+                streams[j].sync()
+                results[j] = download(x, buf)
+                if not np.array_equal(results[j], want[j]):
+                    errors.append((j, rep))
+        except Exception as e:  # reported below, on the main thread
+            errors.append(e)
+
+    def retune():
+        c = 1
+        while not done.is_set():
+            set_tuning(libs, k, c)
+            c = c % 3 + 1
+
+    tuner = threading.Thread(target=retune)
+    workers = [threading.Thread(target=transform, args=(j,)) for j in range(4)]
+    print("ntt concurrent transforms: 4 threads + tuner", flush=True)
+    tuner.start()
+    for t in workers:
+        t.start()
+    for t in workers:
+        t.join()
+    done.set()
+    tuner.join()
+    set_tuning(libs, k, 1)
+    assert not errors, errors
