@@ -266,6 +266,28 @@ def test_workload_2_20_other_windows(libs, stream, window):
     assert list(run_device(libs, stream, pb, ss, window)) == list(closed_form(ks, ss)), window
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 17])
+def test_reference_abi_tiny(libs, stream, n):
+    """The drop-in msm on a handful of points, in one part and in up to four
+    (more parts than points leaves some empty), two batches in one run, the
+    second all zeros: against best_multiexp, the zero batch the identity."""
+    ks = random_elements(n, 9600 + n)
+    pb = gpu_bases(libs, stream, ks)
+    host_pts = np.ascontiguousarray(pb.download(np.uint64, n * 8).reshape(n, 8))
+    scal = [random_elements(n, 9700 + n), np.zeros((n, 4), dtype=np.uint64)]
+    for parts in (1, min(4, n + 1)):
+        cfg = MsmConfig(window_size=12, target_window=22, parts=parts, batch_per_run=2)
+        op = MSM(libs, cfg)
+        size = op.get_buffer_size(n)[0]
+        buf = hip.DeviceBuffer(size)
+        res = [Point(), Point()]
+        var = [PointArray(host_pts.ctypes.data, n)] + [ScalarArray(n, s.ctypes.data, "cpu") for s in scal]
+        print(f"msm tiny n={n} parts={parts}", flush=True)
+        op.get_fn().f([GpuBuffer(buf.addr, size)] + res, var, lambda c: c)
+        assert list(res[0].data) == list(orc.best_multiexp(scal[0], host_pts)), parts
+        assert list(res[1].data) == [0] * 8, parts
+
+
 def test_reference_abi_concurrent_calls(libs, stream):
     """Three threads call the drop-in msm at once (ctypes drops the GIL for the
     call), each with its own configuration, device buffer and host arrays:
