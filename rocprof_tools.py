@@ -115,7 +115,7 @@ def summarize(fetch_csv: str, write_csv: str, out_json: str, out_md: str, log_n:
         json.dump(out, f, indent=1, sort_keys=True)
     with open(out_md, "w") as f:
         f.write(f"# HBM traffic per launch (NTT 2^{log_n}, rocprofv3 FETCH_SIZE / WRITE_SIZE)\n\n")
-        f.write("Commands (repository root on the GPU box; one counter per run):\n\n```\n")
+        f.write("Commands (from the repository root on an MI355X; one counter per run):\n\n```\n")
         f.write("cd /tmp && export TMPDIR=/tmp && export PYTHONPATH=$REPO/zkpoly-compiler_amd\n")
         for c, d in (("FETCH_SIZE", "pmc_fetch"), ("WRITE_SIZE", "pmc_write")):
             f.write(f"rocprofv3 --pmc {c} --output-format csv -d $REPO/gpurun_out/{d} -o run -- "
@@ -149,7 +149,7 @@ def summarize_valu(ntt_csv: str, msm_csv: str, out_json: str, out_md: str) -> di
         json.dump(out, f, indent=1, sort_keys=True)
     with open(out_md, "w") as f:
         f.write("# VALU busy / utilisation per kernel (rocprofv3 derived counters)\n\n")
-        f.write("Commands (repository root on the GPU box):\n\n```\ncd /tmp && export TMPDIR=/tmp\n")
+        f.write("Commands (from the repository root on an MI355X):\n\n```\ncd /tmp && export TMPDIR=/tmp\n")
         f.write("rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d $REPO/gpurun_out/pmc_valu_ntt -o run "
                 "-- python3 -u $REPO/bench.py --quick --no-cpu --steps 3 --warmup 1\n")
         f.write("rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d $REPO/gpurun_out/pmc_valu_msm -o run "
@@ -168,7 +168,7 @@ def summarize_trace(trace_dir: str, title: str, command: str, out_md: str, tail_
                     tail_n: int = 0) -> None:
     with open(f"{trace_dir}/run_kernel_stats.csv") as f:
         stats = list(csv.DictReader(f))
-    lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
+    lines = [f"# {title}", "", "Command (from the repository root on an MI355X):", "", "```", command, "```", "",
              "## Kernel stats (all dispatches of the run)", "",
              "| kernel | calls | avg ms | min ms | max ms | % time |", "|---|---|---|---|---|---|"]
     for r in stats:
@@ -245,7 +245,7 @@ def summarize_timeline(trace_dir: str, title: str, command: str, out_md: str) ->
         cbusy, cu = _union(h2d)
         kbusy, ku = _union(ker)
         rows.append((c, parts, len(h2d), (t1 - t0) / 1e6, cbusy / 1e6, kbusy / 1e6, _overlap(cu, ku) / 1e6))
-    lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
+    lines = [f"# {title}", "", "Command (from the repository root on an MI355X):", "", "```", command, "```", "",
              "## End-to-end batches (host inputs copied in, kernels, window sums back)", "",
              "The trace is cut after every copy of a batch's window sums to the host; a batch that shares a "
              "run with others carries the run's copies in its first piece.  `copy busy`: time at least one "
@@ -277,7 +277,7 @@ def summarize_counter(csv_path: str, counter: str, title: str, command: str, out
             name = r["Kernel_Name"].split("(")[0].split("::")[-1]
             vals[(name, int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
             seq.append((int(r["Dispatch_Id"]), name, int(r["Grid_Size"]), float(r["Counter_Value"])))
-    lines = [f"# {title}", "", "Command (repository root on the GPU box):", "", "```", command, "```", "",
+    lines = [f"# {title}", "", "Command (from the repository root on an MI355X):", "", "```", command, "```", "",
              f"Mean `{counter}` per dispatch, by kernel and grid size (threads).", "",
              "| kernel | grid | dispatches | mean |", "|---|---|---|---|"]
     for (name, grid), v in sorted(vals.items(), key=lambda kv: (-sum(kv[1]) / len(kv[1]), kv[0])):

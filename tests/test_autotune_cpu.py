@@ -15,8 +15,8 @@ def test_tune_keeps_argmin_per_size_and_all_timings():
 
 
 def test_tune_keeps_each_candidates_best_round():
-    """A candidate measured slowly once (clock ramp-up) still wins on its best round."""
-    seq = iter([9.0, 3.3, 3.4, 3.1, 3.3, 3.4])  # round 1: c1 slow; round 2: c1 fastest
+    """A candidate measured slowly once (clock ramp-up) still wins on its best sweep."""
+    seq = iter([9.0, 3.3, 3.4, 3.1, 3.3, 3.4])  # first sweep: c1 slow; second sweep: c1 fastest
     t = at.tune("ntt", [24], [1, 2, 3], lambda k, c: next(seq))
     assert t.ntt == {24: 1} and t.timings["ntt/24/1"] == 3.1
     m = at.tune("msm", [20], [14, 16], lambda k, w: {14: 7.0, 16: 6.5}[w])
