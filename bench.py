@@ -846,7 +846,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
+            f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
             "(multi-GPU sharding is not implemented)",
             file=sys.stderr,
         )

@@ -14,7 +14,7 @@ Two planners here:
   * `reference_choice` restates that model with its A100 coefficients, so the
     choices the reference would make are reproduced (pinned against rows the
     reference's script wrote, tests/golden/msm_cost_model_rows.json);
-  * `plan_msm` is this build's planner for MI355X: the same search space,
+  * `plan_msm` is the planner of this library for MI355X: the same search space,
     each candidate's device memory taken from the library's own sizing call
     (exact, including the stage buffers and the batch_per_run bucket sets),
     and its time from a model of this implementation -- per-entry sort and
@@ -130,7 +130,7 @@ def reference_choice(k: int, n: int, l_bits: int = SCALAR_BITS, p_bits: int = SC
     return best
 
 
-# ---------------------------------------------------------------- this build's planner (MI355X)
+# ---------------------------------------------------------------- the library's planner (MI355X)
 
 # Per-entry cost (ns per (point, window) entry) of the sort plus the bucket
 # accumulation, by window, from zkp_msm_device timings at 2^20 points on one
